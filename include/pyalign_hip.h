@@ -1,0 +1,197 @@
+/*
+ * pyalign_hip.h -- C ABI of libpyalign_hip.so, the MI355X (gfx950) DP
+ * alignment solver.
+ *
+ * This boundary replaces the reference's pybind11 extension module
+ * pyalign.algorithm.<variant>.algorithm (pyalign/algorithm/module.cpp:1-18,
+ * pyalign/algorithm/pyalign.h:12-102).  Every entry point below names the
+ * reference interface it stands in for.  Plain pointers and sizes only; all
+ * pointers are host pointers unless the name says _device.
+ *
+ * Every function returns a status code (PA_OK on success) and leaves a
+ * thread-local message for pa_last_error().  PA_EINVAL maps to Python's
+ * ValueError (std::invalid_argument in the reference), PA_ERUNTIME,
+ * PA_ELENGTH and PA_EHIP to RuntimeError.
+ *
+ * Threading (as the reference, factory.h:705-792): one solver is not
+ * re-entrant; distinct solvers are independent.  Each solver owns one HIP
+ * stream and its device buffers.
+ */
+#ifndef PYALIGN_HIP_H
+#define PYALIGN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PA_OK = 0,
+    PA_EINVAL = 1,   /* std::invalid_argument (factory.h:387-394, 526-533, 606-611) */
+    PA_ERUNTIME = 2, /* std::runtime_error */
+    PA_ELENGTH = 3,  /* exceeded_configured/implementation_length (solver.h:94-152) */
+    PA_EHIP = 4      /* a HIP runtime call failed */
+};
+
+/* enums::Type, Direction, Detail, Count, Locality (pyalign/algorithm/enum.h:6-31) */
+enum { PA_TYPE_ALIGNMENT = 0, PA_TYPE_DTW = 1 };
+enum { PA_MINIMIZE = 0, PA_MAXIMIZE = 1 };
+enum { PA_DETAIL_SCORE = 0, PA_DETAIL_ALIGNMENT = 1, PA_DETAIL_SOLUTION = 2 };
+enum { PA_COUNT_ONE = 0, PA_COUNT_ALL = 1 };
+enum { PA_LOCAL = 0, PA_GLOBAL = 1, PA_SEMIGLOBAL = 2 };
+
+/* Gap cost kinds after GapCost.to_special_case() (pyalign/gaps.py:13-14,
+ * 102-108, 140-143, 181-184; factory.h:33-64). */
+enum { PA_GAP_NONE = 0, PA_GAP_LINEAR = 1, PA_GAP_AFFINE = 2, PA_GAP_GENERAL = 3 };
+
+/* Problem forms (pyalign/problems/instance.py:7-10). */
+enum { PA_FORM_DENSE = 0, PA_FORM_INDEXED = 1, PA_FORM_BINARY = 2, PA_FORM_SERIES = 3 };
+
+/* What a solve produces: the goal of factory.h:1195-1239. */
+enum { PA_GOAL_SCORE = 0, PA_GOAL_ONE = 1, PA_GOAL_ALL = 2 };
+
+/*
+ * The options dict of create_options() (options.h:16-44) with its gap costs
+ * already mapped by GapCostOptions (factory.h:33-64).  gap_s/gap_t hold the
+ * to_special_case() numbers in float64: linear {u, -}; affine {u, v} with
+ * pyalign.gaps' w(k) = u + v*k, i.e. u = open - extend, v = extend.
+ */
+typedef struct pa_options_desc {
+    int32_t solver_type; /* PA_TYPE_* */
+    int32_t batch;       /* options['batch'] */
+    int32_t direction;   /* PA_MINIMIZE / PA_MAXIMIZE */
+    int32_t return_dup;  /* options['return_dup'] */
+    int32_t detail;      /* PA_DETAIL_* (codomain.detail) */
+    int32_t count;       /* PA_COUNT_* (codomain.count) */
+    int32_t locality;    /* PA_LOCAL / PA_GLOBAL / PA_SEMIGLOBAL */
+    int32_t gap_kind_s;  /* PA_GAP_* for s (rows) */
+    int32_t gap_kind_t;  /* PA_GAP_* for t (columns) */
+    int32_t index_bits;  /* 16 for m16 (Options<float,int16_t>), 32 for m32 */
+    double gap_s[2];
+    double gap_t[2];
+} pa_options_desc;
+
+typedef struct pa_options pa_options;
+typedef struct pa_solver pa_solver;
+typedef struct pa_result pa_result;
+
+/* Thread-local text of the last error. */
+const char *pa_last_error(void);
+
+/* Number of HIP devices visible (0 when none); never fails. */
+int pa_device_count(void);
+
+/* create_options(dict) -> Options (pyalign.h:12-15, options.h:51-60). */
+int pa_create_options(const pa_options_desc *desc, pa_options **out);
+void pa_options_free(pa_options *opts);
+
+/* create_solver(max_len_s, max_len_t, options) -> Solver
+ * (factory.h:1268-1278).  The limits fix the solver's border vectors exactly
+ * as the reference's MatrixFactory/solver constructors do. */
+int pa_create_solver(int64_t max_len_s, int64_t max_len_t, const pa_options *opts,
+                     pa_solver **out);
+void pa_solver_free(pa_solver *solver);
+
+/* Solver.batch_size (pyalign.h:24, factory.h:422): lanes per call. */
+int pa_solver_batch_size(const pa_solver *solver);
+
+/* Solver.solve_for_{score,alignment,solution,...}: the dense MatrixForm
+ * (factory.h:540-580).  sim is float32[Ls][Lt][B] (lane innermost), len is
+ * int32[2][B] (row 0 = len_s, row 1 = len_t). */
+int pa_solve_dense(pa_solver *solver, const float *sim, int64_t Ls, int64_t Lt, int64_t B,
+                   const int32_t *len, pa_result **out);
+
+/* Solver.solve_indexed_for_*: IndexedMatrixForm (factory.h:582-631).
+ * a is uint32[B][La], b uint32[B][Lb], sim float32[K0][K1]. */
+int pa_solve_indexed(pa_solver *solver, const uint32_t *a, int64_t La, const uint32_t *b,
+                     int64_t Lb, int64_t B, const float *sim, int64_t K0, int64_t K1,
+                     const int32_t *len, pa_result **out);
+
+/* Solver.solve_binary_for_*: BinaryMatrixForm (factory.h:633-680). */
+int pa_solve_binary(pa_solver *solver, const uint32_t *a, int64_t La, const uint32_t *b,
+                    int64_t Lb, int64_t B, float eq, float ne, const int32_t *len,
+                    pa_result **out);
+
+/*
+ * Ragged batch: N problems of individual lengths in one call, the whole bag
+ * at once instead of the reference's per-B Python loop (solve.py:554-560).
+ * Batching is result-neutral (each lane's cells depend only on its own
+ * region, solver.h:2593-2705), so results equal the per-batch calls.
+ *
+ * Indexed form: codes_s/codes_t are uint8 symbol codes (< K <= 255),
+ *   sequence k occupies codes_s[off_s[k] .. off_s[k]+len_s[k]).
+ * Binary form: codes are uint32 values compared for equality.
+ * Dense form: dense + dense_off[k] is problem k's row-major [len_s][len_t].
+ * Series form (DTW): codes_s/codes_t are float32 series of `dim` columns,
+ *   offsets in rows; cost = sum over dims of squared difference.
+ * Lengths and offsets are host arrays; the payload pointers are host
+ * pointers, or device pointers when on_device = 1 (inputs resident in HBM).
+ */
+typedef struct pa_ragged {
+    int64_t n;
+    int32_t form;       /* PA_FORM_* */
+    int32_t on_device;  /* payload pointers are device pointers */
+    const void *codes_s;
+    const void *codes_t;
+    const int64_t *off_s;
+    const int64_t *off_t;
+    const int32_t *len_s;
+    const int32_t *len_t;
+    const float *table; /* indexed: float32[K][K] (host) */
+    int32_t K;
+    int32_t dim;        /* series dimension */
+    float eq, ne;       /* binary */
+    const float *dense; /* dense payload */
+    const int64_t *dense_off;
+} pa_ragged;
+
+/* Scheduling of a ragged batch. */
+enum {
+    PA_SCHED_AUTO = 0,  /* cost-sorted work queue, per-bucket strip height (autotuned) */
+    PA_SCHED_PLAIN = 1  /* the reference's Morton batches (instance.py:230-275) of 64,
+                           one wavefront each, fixed strip height */
+};
+
+int pa_solve_ragged(pa_solver *solver, const pa_ragged *batch, int32_t schedule,
+                    pa_result **out);
+
+/* Score-only ragged solve writing float32 scores to a device buffer on the
+ * solver's stream, without host synchronisation; for callers whose inputs
+ * already sit in HBM.  Pair with pa_solver_sync. */
+int pa_solve_ragged_scores_device(pa_solver *solver, const pa_ragged *batch, int32_t schedule,
+                                  float *scores_device);
+int pa_solver_sync(pa_solver *solver);
+
+/* Timing of the last solve's DP kernel launches (HIP events on the solver's
+ * stream): total milliseconds and launch count. */
+int pa_solver_last_kernel_ms(const pa_solver *solver, double *ms, int32_t *launches,
+                             double *cells);
+
+/* ---- results (the tuple of per-lane objects, factory.h:699-808) ------ */
+
+int64_t pa_result_count(const pa_result *r);
+/* Scores of all problems; the worst value for a problem with no seed
+ * (Solver::score, solver.h:2593-2608). */
+int pa_result_scores(const pa_result *r, float *out);
+/* 1 if problem k has a traceback seed (an optimal alignment exists). */
+int pa_result_has_path(const pa_result *r, int64_t k);
+/* Path of problem k, seed first (build_path, solver.h:858-935): n points. */
+int64_t pa_result_path_len(const pa_result *r, int64_t k);
+int pa_result_path(const pa_result *r, int64_t k, int32_t *uv);
+/* Alignment of problem k (factory.h:130-193): s_to_t int32[len_s],
+ * t_to_s int32[len_t], -1 where unaligned. */
+int pa_result_alignment(const pa_result *r, int64_t k, int32_t *s_to_t, int32_t *t_to_s);
+/* All-optimal iteration (TracebackIterators<true>, solver.h:1568-1743):
+ * returns 1 and fills score/path for the next alignment of problem k, 0 at
+ * the end.  The path is read with pa_result_iter_path afterwards. */
+int pa_result_iter_next(pa_result *r, int64_t k, float *score);
+int64_t pa_result_iter_path_len(const pa_result *r, int64_t k);
+int pa_result_iter_path(const pa_result *r, int64_t k, int32_t *uv);
+void pa_result_free(pa_result *r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYALIGN_HIP_H */

@@ -1,0 +1,525 @@
+"""The native solver module, backed by libpyalign_hip.so through ctypes.
+
+This module stands in for the reference's pybind11 extension
+``pyalign.algorithm.<variant>.algorithm`` (pyalign/algorithm/module.cpp:1-18,
+pyalign/algorithm/pyalign.h:12-102) and exposes the same names: the enums
+``Type``, ``Direction``, ``Detail``, ``Count``, ``Locality``; the
+submodules ``m16`` / ``m32`` with ``create_options`` and ``create_solver``;
+the ``Solver`` methods ``solve_{,indexed_,binary_}for_{score,alignment,
+alignment_iterator,solution,solution_iterator}``; and the result classes
+``Alignment``, ``AlignmentIterator``, ``Solution``, ``SolutionIterator``,
+``Algorithm``.  All DP work runs in the HIP kernels.
+"""
+
+import ctypes
+import enum
+import types
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+class Type(enum.IntEnum):  # enum.h:6-9
+    ALIGNMENT = 0
+    DTW = 1
+
+
+class Direction(enum.IntEnum):  # enum.h:11-14
+    MINIMIZE = 0
+    MAXIMIZE = 1
+
+
+class Detail(enum.IntEnum):  # enum.h:16-20
+    SCORE = 0
+    ALIGNMENT = 1
+    SOLUTION = 2
+
+
+class Count(enum.IntEnum):  # enum.h:22-25
+    ONE = 0
+    ALL = 1
+
+
+class Locality(enum.IntEnum):  # enum.h:27-31
+    LOCAL = 0
+    GLOBAL = 1
+    SEMIGLOBAL = 2
+
+
+class Algorithm:
+    """AlgorithmMetaData (solver.h:154-177) as exposed by pyalign.h:235-241."""
+
+    def __init__(self, name, runtime, memory):
+        self._name, self._runtime, self._memory = name, runtime, memory
+
+    @property
+    def name(self):
+        return self._name
+
+    @property
+    def runtime(self):
+        return self._runtime
+
+    @property
+    def memory(self):
+        return self._memory
+
+
+def _algorithm_for(opts):
+    # solver.h:2799-2802, 2914, 3107, 3203
+    if opts.solver == Type.DTW:
+        return Algorithm("DTW", "n^2", "n^2")
+    if opts.gap_kind == "affine":
+        return Algorithm("Gotoh", "n^2", "n^2")
+    return Algorithm("Needleman-Wunsch" if opts.locality == Locality.GLOBAL else "Smith-Waterman", "n^2", "n^2")
+
+
+def _gap_spec(gap):
+    """GapCostOptions (factory.h:33-64): (kind, [u, v]) from to_special_case()."""
+    if gap is None:
+        return _lib.PA_GAP_NONE, (0.0, 0.0)
+    special = gap.to_special_case()
+    if "affine" in special:
+        u, v = special["affine"]
+        return _lib.PA_GAP_AFFINE, (float(u), float(v))
+    if "linear" in special:
+        return _lib.PA_GAP_LINEAR, (float(special["linear"]), 0.0)
+    return _lib.PA_GAP_GENERAL, (0.0, 0.0)
+
+
+class Options:
+    """Parsed options dict (options.h:9-117)."""
+
+    def __init__(self, options, index_bits):
+        self._dict = options
+        self.index_bits = index_bits
+        self.solver = Type(int(options["solver"]))
+        self.batch = bool(options["batch"])
+        self.direction = Direction(int(options["direction"]))
+        self.return_dup = bool(options["return_dup"])
+        desc = _lib.OptionsDesc()
+        desc.solver_type = int(self.solver)
+        desc.batch = int(self.batch)
+        desc.direction = int(self.direction)
+        desc.return_dup = int(self.return_dup)
+        desc.index_bits = index_bits
+        self.locality = Locality.LOCAL
+        self.gap_kind = None
+        if self.solver == Type.ALIGNMENT:
+            codomain = options["codomain"]
+            self.detail = Detail(int(codomain.detail))
+            self.count = Count(int(codomain.count))
+            self.locality = Locality(int(options["locality"])) if "locality" in options else Locality.LOCAL
+            gap = options.get("gap_cost")
+            if isinstance(gap, dict):
+                gs, gt = gap.get("s"), gap.get("t")
+            else:
+                gs = gt = gap
+            ks, vs = _gap_spec(gs)
+            kt, vt = _gap_spec(gt)
+            desc.detail, desc.count, desc.locality = int(self.detail), int(self.count), int(self.locality)
+            desc.gap_kind_s, desc.gap_kind_t = ks, kt
+            desc.gap_s[0], desc.gap_s[1] = vs
+            desc.gap_t[0], desc.gap_t[1] = vt
+            self.gap_kind = "affine" if ks == _lib.PA_GAP_AFFINE and kt == _lib.PA_GAP_AFFINE else "linear"
+        else:
+            self.detail, self.count = Detail.SOLUTION, Count.ONE
+        h = ctypes.c_void_p()
+        check(lib().pa_create_options(ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pa_options_free(self._h)
+            self._h = None
+
+    def to_dict(self):
+        return self._dict
+
+
+class Alignment:
+    """Alignment (factory.h:130-193): score and s_to_t / t_to_s maps."""
+
+    def __init__(self, score, s_to_t, t_to_s):
+        self._score = float(score)
+        self._s_to_t = s_to_t
+        self._t_to_s = t_to_s
+
+    @property
+    def score(self):
+        return self._score
+
+    @property
+    def s_to_t(self):
+        return self._s_to_t
+
+    @property
+    def t_to_s(self):
+        return self._t_to_s
+
+
+class Solution:
+    """Solution (factory.h:230-352) of one problem."""
+
+    def __init__(self, score, path, alignment, algorithm, max_degree_1=True, lazy=None):
+        self._score = score
+        self._path = path
+        self._alignment = alignment
+        self._algorithm = algorithm
+        self._max_degree_1 = max_degree_1
+        self._lazy = lazy
+
+    @property
+    def score(self):
+        return self._score
+
+    @property
+    def path(self):
+        return self._path
+
+    @property
+    def alignment(self):
+        return self._alignment
+
+    @property
+    def algorithm(self):
+        return self._algorithm
+
+    @property
+    def traceback_has_max_degree_1(self):
+        return self._max_degree_1
+
+    def _materialised(self):
+        if self._lazy is None:
+            raise NotImplementedError(
+                "Solution.values / traceback matrices are not materialised by the MI355X solver")
+        return self._lazy()
+
+    @property
+    def values(self):
+        return self._materialised()["values"]
+
+    @property
+    def traceback_as_matrix(self):
+        return self._materialised()["traceback_as_matrix"]
+
+    @property
+    def traceback_as_edges(self):
+        return self._materialised()["traceback_as_edges"]
+
+
+class _Result:
+    """Owns one pa_result handle."""
+
+    def __init__(self, h, n, index_dtype, len_s, len_t):
+        self.h = h
+        self.n = n
+        self.index_dtype = index_dtype
+        self.len_s = np.asarray(len_s, dtype=np.int64)
+        self.len_t = np.asarray(len_t, dtype=np.int64)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pa_result_free(self.h)
+            self.h = None
+
+    def scores(self):
+        out = np.empty(self.n, dtype=np.float32)
+        if self.n:
+            check(lib().pa_result_scores(self.h, out.ctypes.data))
+        return out
+
+    def path(self, k):
+        m = lib().pa_result_path_len(self.h, k)
+        if m < 0:
+            return None
+        uv = np.empty((m, 2), dtype=np.int32)
+        check(lib().pa_result_path(self.h, k, uv.ctypes.data))
+        return uv.astype(self.index_dtype)
+
+    def alignment(self, k, score):
+        ls, lt = int(self.len_s[k]), int(self.len_t[k])
+        s_to_t = np.empty(ls, dtype=np.int32)
+        t_to_s = np.empty(lt, dtype=np.int32)
+        check(lib().pa_result_alignment(self.h, k, s_to_t.ctypes.data, t_to_s.ctypes.data))
+        return Alignment(score, s_to_t.astype(self.index_dtype), t_to_s.astype(self.index_dtype))
+
+    def iter_next(self, k):
+        score = ctypes.c_float()
+        r = lib().pa_result_iter_next(self.h, k, ctypes.byref(score))
+        if r < 0:
+            check(-r)
+        if r == 0:
+            return None
+        m = lib().pa_result_iter_path_len(self.h, k)
+        uv = np.empty((m, 2), dtype=np.int32)
+        check(lib().pa_result_iter_path(self.h, k, uv.ctypes.data))
+        return float(np.float32(score.value)), uv.astype(self.index_dtype)
+
+
+class AlignmentIterator:
+    """AlignmentIterator (factory.h:195-216): next() returns None at the end."""
+
+    def __init__(self, result, k):
+        self._r, self._k = result, k
+
+    def next(self):
+        x = self._r.iter_next(self._k)
+        if x is None:
+            return None
+        return self._r.alignment(self._k, x[0])
+
+
+class SolutionIterator:
+    """SolutionIterator (factory.h:354-385); solutions carry no algorithm
+    record, as in the reference's iterator path (solver.h:1881)."""
+
+    def __init__(self, result, k):
+        self._r, self._k = result, k
+
+    def next(self):
+        x = self._r.iter_next(self._k)
+        if x is None:
+            return None
+        score, path = x
+        return Solution(score, path, self._r.alignment(self._k, score), None, max_degree_1=False)
+
+
+def _i32(x):
+    return np.ascontiguousarray(x, dtype=np.int32)
+
+
+class Solver:
+    """Solver<Value, Index> (factory.h:414-508) on the MI355X."""
+
+    def __init__(self, h, opts, max_len_s, max_len_t):
+        self._h = h
+        self._opts = opts
+        self._index_dtype = np.int16 if opts.index_bits == 16 else np.int32
+        self.max_len_s, self.max_len_t = max_len_s, max_len_t
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pa_solver_free(self._h)
+            self._h = None
+
+    @property
+    def options(self):
+        return self._opts.to_dict()
+
+    @property
+    def batch_size(self):
+        return lib().pa_solver_batch_size(self._h)
+
+    # ---- shared result conversion ------------------------------------
+    def _wrap(self, h, len_s, len_t, kind):
+        n = lib().pa_result_count(h)
+        r = _Result(h, n, self._index_dtype, len_s, len_t)
+        scores = r.scores()
+        if kind == "score":
+            return scores
+        if kind == "alignment":
+            return tuple(r.alignment(k, scores[k]) for k in range(n))
+        if kind == "solution":
+            alg = _algorithm_for(self._opts)
+            out = []
+            for k in range(n):
+                path = r.path(k)
+                if path is None:
+                    out.append(Solution(None, None, None, alg))
+                else:
+                    out.append(Solution(float(scores[k]), path, r.alignment(k, scores[k]), alg))
+            return tuple(out)
+        if kind == "alignment_iterator":
+            return tuple(AlignmentIterator(r, k) for k in range(n))
+        if kind == "solution_iterator":
+            return tuple(SolutionIterator(r, k) for k in range(n))
+        raise ValueError(kind)
+
+    def _dense(self, sim, length, kind):
+        sim = np.ascontiguousarray(sim, dtype=np.float32)
+        length = _i32(length)
+        if sim.ndim != 3 or length.shape != (2, sim.shape[2]):
+            raise ValueError(f"m_length has shape {length.shape}, expected (2, {sim.shape[2] if sim.ndim == 3 else '?'})")
+        h = ctypes.c_void_p()
+        check(lib().pa_solve_dense(self._h, sim.ctypes.data, sim.shape[0], sim.shape[1], sim.shape[2],
+                                   length.ctypes.data, ctypes.byref(h)))
+        return self._wrap(h, length[0], length[1], kind)
+
+    def _indexed(self, a, b, sim, length, kind):
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        sim = np.ascontiguousarray(sim, dtype=np.float32)
+        length = _i32(length)
+        if length.shape != (2, a.shape[0]):
+            raise ValueError(f"m_length has shape {length.shape}, expected (2, {a.shape[0]})")
+        if b.shape[0] != a.shape[0]:
+            raise ValueError(f"problem of batch size {b.shape[0]} given to solver that was configured to "
+                             f"batch size {self.batch_size}")
+        h = ctypes.c_void_p()
+        check(lib().pa_solve_indexed(self._h, a.ctypes.data, a.shape[1], b.ctypes.data, b.shape[1], a.shape[0],
+                                     sim.ctypes.data, sim.shape[0], sim.shape[1], length.ctypes.data,
+                                     ctypes.byref(h)))
+        return self._wrap(h, length[0], length[1], kind)
+
+    def _binary(self, a, b, eq, ne, length, kind):
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        length = _i32(length)
+        if length.shape != (2, a.shape[0]):
+            raise ValueError(f"m_length has shape {length.shape}, expected (2, {a.shape[0]})")
+        h = ctypes.c_void_p()
+        check(lib().pa_solve_binary(self._h, a.ctypes.data, a.shape[1], b.ctypes.data, b.shape[1], a.shape[0],
+                                    float(eq), float(ne), length.ctypes.data, ctypes.byref(h)))
+        return self._wrap(h, length[0], length[1], kind)
+
+    # ---- the reference's 15 entry points (pyalign.h:24-41) -----------
+    def solve_for_score(self, sim, length):
+        return self._dense(sim, length, "score")
+
+    def solve_indexed_for_score(self, a, b, sim, length):
+        return self._indexed(a, b, sim, length, "score")
+
+    def solve_binary_for_score(self, a, b, eq, ne, length):
+        return self._binary(a, b, eq, ne, length, "score")
+
+    def solve_for_alignment(self, sim, length):
+        return self._dense(sim, length, "alignment")
+
+    def solve_for_alignment_iterator(self, sim, length):
+        return self._dense(sim, length, "alignment_iterator")
+
+    def solve_indexed_for_alignment(self, a, b, sim, length):
+        return self._indexed(a, b, sim, length, "alignment")
+
+    def solve_indexed_for_alignment_iterator(self, a, b, sim, length):
+        return self._indexed(a, b, sim, length, "alignment_iterator")
+
+    def solve_binary_for_alignment(self, a, b, eq, ne, length):
+        return self._binary(a, b, eq, ne, length, "alignment")
+
+    def solve_binary_for_alignment_iterator(self, a, b, eq, ne, length):
+        return self._binary(a, b, eq, ne, length, "alignment_iterator")
+
+    def solve_for_solution(self, sim, length):
+        return self._dense(sim, length, "solution")
+
+    def solve_for_solution_iterator(self, sim, length):
+        return self._dense(sim, length, "solution_iterator")
+
+    def solve_indexed_for_solution(self, a, b, sim, length):
+        return self._indexed(a, b, sim, length, "solution")
+
+    def solve_indexed_for_solution_iterator(self, a, b, sim, length):
+        return self._indexed(a, b, sim, length, "solution_iterator")
+
+    def solve_binary_for_solution(self, a, b, eq, ne, length):
+        return self._binary(a, b, eq, ne, length, "solution")
+
+    def solve_binary_for_solution_iterator(self, a, b, eq, ne, length):
+        return self._binary(a, b, eq, ne, length, "solution_iterator")
+
+    # ---- whole-bag entry (no reference counterpart: one call per bag) --
+    def solve_ragged(self, kind, form, len_s, len_t, schedule=_lib.PA_SCHED_AUTO, **payload):
+        """Solve n problems of individual lengths in one call.
+
+        form: "indexed" (codes_s/codes_t uint8, off_s/off_t, table KxK),
+        "binary" (uint32 codes, eq, ne), "dense" (dense float32 flat,
+        dense_off) or "series" (float32 rows x dim).  kind as in _wrap.
+        """
+        rg, keep = make_ragged(form, len_s, len_t, **payload)
+        h = ctypes.c_void_p()
+        check(lib().pa_solve_ragged(self._h, ctypes.byref(rg), schedule, ctypes.byref(h)))
+        del keep
+        return self._wrap(h, len_s, len_t, kind)
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_double()
+        launches = ctypes.c_int32()
+        cells = ctypes.c_double()
+        check(lib().pa_solver_last_kernel_ms(self._h, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(cells)))
+        return ms.value, launches.value, cells.value
+
+
+_FORMS = {"dense": _lib.PA_FORM_DENSE, "indexed": _lib.PA_FORM_INDEXED,
+          "binary": _lib.PA_FORM_BINARY, "series": _lib.PA_FORM_SERIES}
+
+
+def make_ragged(form, len_s, len_t, codes_s=None, codes_t=None, off_s=None, off_t=None, table=None,
+                eq=0.0, ne=0.0, dense=None, dense_off=None, dim=0, on_device=False):
+    """Build a pa_ragged descriptor; returns (struct, arrays to keep alive).
+
+    With on_device=True the payload arguments are device pointers (ints).
+    """
+    keep = []
+
+    def arr(x, dt):
+        a = np.ascontiguousarray(x, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    def payload(x, dt):
+        if x is None:
+            return None
+        if on_device:
+            return int(x)
+        return arr(x, dt)
+
+    rg = _lib.Ragged()
+    ls = np.ascontiguousarray(len_s, dtype=np.int32)
+    lt = np.ascontiguousarray(len_t, dtype=np.int32)
+    keep += [ls, lt]
+    rg.n = len(ls)
+    rg.form = _FORMS[form]
+    rg.on_device = int(bool(on_device))
+    rg.len_s, rg.len_t = ls.ctypes.data, lt.ctypes.data
+    if off_s is not None:
+        rg.off_s = arr(off_s, np.int64)
+        rg.off_t = arr(off_t, np.int64)
+    if form == "indexed":
+        rg.codes_s = payload(codes_s, np.uint8)
+        rg.codes_t = payload(codes_t, np.uint8)
+        t = np.ascontiguousarray(table, dtype=np.float32)
+        keep.append(t)
+        rg.table = t.ctypes.data
+        rg.K = t.shape[0]
+    elif form == "binary":
+        rg.codes_s = payload(codes_s, np.uint32)
+        rg.codes_t = payload(codes_t, np.uint32)
+        rg.eq, rg.ne = float(eq), float(ne)
+    elif form == "series":
+        rg.codes_s = payload(codes_s, np.float32)
+        rg.codes_t = payload(codes_t, np.float32)
+        rg.dim = int(dim)
+    elif form == "dense":
+        rg.dense = payload(dense, np.float32)
+        rg.dense_off = arr(dense_off, np.int64)
+    return rg, keep
+
+
+def _make_module(index_bits):
+    mod = types.SimpleNamespace()
+
+    def create_options(options):
+        return Options(dict(options), index_bits)
+
+    def create_solver(max_len_s, max_len_t, options):
+        h = ctypes.c_void_p()
+        check(lib().pa_create_solver(int(max_len_s), int(max_len_t), options._h, ctypes.byref(h)))
+        return Solver(h, options, int(max_len_s), int(max_len_t))
+
+    mod.create_options = create_options
+    mod.create_solver = create_solver
+    mod.Solver = Solver
+    mod.Alignment = Alignment
+    mod.AlignmentIterator = AlignmentIterator
+    mod.Solution = Solution
+    mod.SolutionIterator = SolutionIterator
+    mod.Options = Options
+    return mod
+
+
+m16 = _make_module(16)  # Options<float, int16_t> (module.cpp:10-11)
+m32 = _make_module(32)  # Options<float, int32_t> (module.cpp:13-14)

@@ -1,0 +1,1308 @@
+// pa_api.cpp -- the C ABI of libpyalign_hip.so (include/pyalign_hip.h).
+//
+// Host side of the solver: options and solver objects mirroring the
+// reference's pybind11 surface (pyalign/algorithm/pyalign.h:12-102,
+// factory.h:414-508, 1243-1278), the border vectors of the reference's
+// solver constructors, the scheduler that turns a ragged bag of problems
+// into wavefront groups (the autotuned work queue, or the reference's
+// Morton batches for the plain form), the launches, and the result objects
+// (scores, one-optimal paths, all-optimal enumeration over the GPU's
+// traceback bits).
+#include "pyalign_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pa_internal.h"
+
+using namespace pa;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+#define HIPCHK(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) fail(PA_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        return PA_OK;
+    } catch (const Error &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "out of host memory";
+        return PA_ERUNTIME;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return PA_ERUNTIME;
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void ensure(size_t bytes) {
+        if (bytes <= cap) return;
+        if (p) HIPCHK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        const size_t c = std::max<size_t>(bytes + bytes / 4, 256);
+        HIPCHK(hipMalloc(&p, c));
+        cap = c;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+inline float worst(int dir) {
+    return dir == PA_MAXIMIZE ? -std::numeric_limits<float>::infinity() : std::numeric_limits<float>::infinity();
+}
+
+// AffineCost::vector (solver.h:2876-2880): linspace<float>(0, (n-1)*u, n) + v,
+// w[0] = 0, with xtensor's double-precision generator (start + k*step,
+// stop at k = n-1).  Equals fl32(fl32(k*u) + v) whenever (n-1)*u is exact.
+std::vector<float> affine_vector(float u, float v, int64_t n) {
+    std::vector<float> w(static_cast<size_t>(n));
+    const float stop = float(n - 1) * u;
+    const double step = double(stop - 0.0f) / std::max(1.0, double(n - 1));
+    for (int64_t k = 0; k < n; k++) {
+        const float lin = (n > 1 && k == n - 1) ? stop : float(0.0 + double(k) * step);
+        w[size_t(k)] = lin + v;
+    }
+    w[0] = 0.0f;
+    return w;
+}
+
+uint64_t part1by1(uint64_t x) {
+    x &= 0xffffffffull;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+// Morton code of a problem shape (pymorton.interleave2: x in the even bits).
+uint64_t morton(uint64_t x, uint64_t y) { return part1by1(x) | (part1by1(y) << 1); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// options / solver objects
+// ---------------------------------------------------------------------------
+
+struct pa_options {
+    pa_options_desc d{};
+    int type = 0, dir = 0, loc = 0, goal = 0, gap = 0;
+    float lin_s = 0, lin_t = 0;
+    float aff_u_s = 0, aff_v_s = 0, aff_u_t = 0, aff_v_t = 0;  // AffineCost(u=extend, v=open-extend)
+    bool remove_dup = true;
+    int index_bits = 16;
+};
+
+struct pa_solver {
+    pa_options opt;
+    int64_t lim_s = 0, lim_t = 0;
+    std::vector<float> border_s, border_t;
+    float go_s = 0, ge_s = 0, go_t = 0, ge_t = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cus = 256;
+    bool has_device = false;
+    DevBuf b_border_s, b_border_t, b_probs, b_groups, b_queue, b_cs, b_ct, b_table, b_dense, b_rowbuf;
+    DevBuf b_scores, b_seeds, b_tb, b_path, b_path_off, b_path_len, b_dvals;
+    double last_ms = 0.0;
+    int last_launches = 0;
+    double last_cells = 0.0;
+
+    ~pa_solver() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// One problem's all-optimal enumeration state (TracebackIterators<true>,
+// solver.h:1568-1743).
+struct IterState {
+    struct Entry {
+        float val;
+        int32_t u, v;
+        int64_t path_len;
+    };
+    std::vector<Entry> stack;
+    std::vector<std::pair<int32_t, int32_t>> path;
+    std::set<std::vector<std::pair<int32_t, int32_t>>> emitted;
+    std::vector<std::pair<int32_t, int32_t>> cur_path;
+    bool started = false;
+};
+
+}  // namespace
+
+struct pa_result {
+    int64_t n = 0;
+    int dir = 0, loc = 0, gap = 0, goal = 0;
+    int index_bits = 16;
+    bool remove_dup = true;
+    std::vector<float> scores;
+    std::vector<int32_t> seeds;
+    std::vector<int32_t> len_s, len_t;
+    // one-optimal paths
+    std::vector<int64_t> path_off;
+    std::vector<int32_t> path_len, paths;
+    // all-optimal: traceback bits and local D values
+    std::vector<uint32_t> tb;
+    std::vector<float> dvals;
+    std::vector<Prob> probs;  // indexed by problem
+    std::vector<std::unique_ptr<IterState>> iters;
+};
+
+namespace {
+
+void make_borders(pa_solver &s) {
+    const pa_options &o = s.opt;
+    const int64_t ns = s.lim_s + 1, nt = s.lim_t + 1;
+    const float sgn = o.dir == PA_MINIMIZE ? 1.0f : -1.0f;
+    s.border_s.assign(size_t(ns), 0.0f);
+    s.border_t.assign(size_t(nt), 0.0f);
+    if (o.type == PA_TYPE_DTW) {
+        // DynamicTimeSolver: every border cell is the worst value except
+        // D(-1,-1) = 0 (solver.h:3205-3210)
+        std::fill(s.border_s.begin(), s.border_s.end(), worst(o.dir));
+        std::fill(s.border_t.begin(), s.border_t.end(), worst(o.dir));
+        s.border_s[0] = s.border_t[0] = 0.0f;
+        return;
+    }
+    if (o.loc != PA_GLOBAL) return;  // Local/Semiglobal fill 0 (solver.h:2021-2029, 2253-2261)
+    if (o.gap == GAP_LIN) {
+        // arange<index>(0, n) * gap * sgn (solver.h:2806-2814)
+        for (int64_t k = 0; k < ns; k++) s.border_s[size_t(k)] = (float(k) * o.lin_s) * sgn;
+        for (int64_t k = 0; k < nt; k++) s.border_t[size_t(k)] = (float(k) * o.lin_t) * sgn;
+    } else {
+        // w(k) * sgn for D's border column/row (solver.h:2943-2951)
+        const std::vector<float> ws = affine_vector(o.aff_u_s, o.aff_v_s, ns);
+        const std::vector<float> wt = affine_vector(o.aff_u_t, o.aff_v_t, nt);
+        for (int64_t k = 0; k < ns; k++) s.border_s[size_t(k)] = ws[size_t(k)] * sgn;
+        for (int64_t k = 0; k < nt; k++) s.border_t[size_t(k)] = wt[size_t(k)] * sgn;
+    }
+    // D(-1,-1) is written last by the row initialisation
+    s.border_s[0] = s.border_t[0];
+}
+
+void ensure_device(pa_solver &s) {
+    if (s.has_device) return;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        fail(PA_EHIP, "no HIP device is available: the pyalign_amd solver runs only on an AMD GPU");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    s.cus = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreate(&s.stream));
+    HIPCHK(hipEventCreate(&s.ev0));
+    HIPCHK(hipEventCreate(&s.ev1));
+    s.b_border_s.ensure(s.border_s.size() * 4);
+    s.b_border_t.ensure(s.border_t.size() * 4);
+    HIPCHK(hipMemcpy(s.b_border_s.p, s.border_s.data(), s.border_s.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s.b_border_t.p, s.border_t.data(), s.border_t.size() * 4, hipMemcpyHostToDevice));
+    s.has_device = true;
+}
+
+// ---------------------------------------------------------------------------
+// scheduling
+// ---------------------------------------------------------------------------
+
+struct Plan {
+    std::vector<Prob> probs;    // in group order
+    std::vector<Group> groups;  // per launch, concatenated
+    std::vector<std::pair<int, std::pair<int, int>>> launches;  // R, [first group, end group)
+    int64_t rowbuf = 0;         // float2 elements
+    int64_t tb_words = 0;
+    int64_t dvals = 0;
+    double cells = 0.0;
+};
+
+int ilog2(int x) {
+    int r = 0;
+    while ((1 << r) < x) r++;
+    return r;
+}
+
+// Choose (L, R) per problem: the most efficient layout whose per-lane
+// serial work stays under the cap that keeps >= 4 wavefronts of work per
+// SIMD; the largest problems fall back to the widest segment.
+void choose_layout(int64_t ls, int64_t lt, const std::vector<int> &Rs, double cap, int &L_out, int &R_out) {
+    double best_eff = -1.0, best_serial = 1e300;
+    int bL = 64, bR = Rs.back();
+    double fallback_serial = 1e300;
+    int fL = 64, fR = Rs.back();
+    for (int L = 1; L <= 64; L <<= 1) {
+        for (int R : Rs) {
+            const int64_t rps = int64_t(L) * R;
+            const int64_t nst = (ls + rps - 1) / rps;
+            const double steps = double(lt + L - 1);
+            const double serial = double(nst) * steps * R;
+            const double eff = double(ls) * double(lt) / (double(nst * rps) * steps);
+            if (serial < fallback_serial || (serial == fallback_serial && L > fL)) {
+                fallback_serial = serial;
+                fL = L;
+                fR = R;
+            }
+            if (serial > cap) continue;
+            if (eff > best_eff + 1e-9 || (std::fabs(eff - best_eff) <= 1e-9 && serial < best_serial)) {
+                best_eff = eff;
+                best_serial = serial;
+                bL = L;
+                bR = R;
+            }
+        }
+    }
+    if (best_eff < 0) {
+        bL = fL;
+        bR = fR;
+    }
+    L_out = bL;
+    R_out = bR;
+}
+
+struct Job {
+    int loc, dir, gap, form, tr;
+    bool fast;
+};
+
+Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, int simds) {
+    std::vector<int> Rs;
+    for (int R : {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32})
+        if (supported_R(job.form, job.tr, job.fast, R)) Rs.push_back(R);
+    if (job.tr == TR_ALL) Rs = {8};
+    // total work at the most efficient layout (no cap) sets the serial cap
+    double total = 0.0;
+    for (int64_t k = 0; k < n; k++) {
+        if (ls[k] <= 0 || lt[k] <= 0) continue;
+        int L, R;
+        choose_layout(ls[k], lt[k], Rs, 1e300, L, R);
+        const int64_t rps = int64_t(L) * R;
+        const int64_t nst = (ls[k] + rps - 1) / rps;
+        total += double(nst) * rps * double(lt[k] + L - 1) / 64.0;
+    }
+    const double cap = std::max(256.0, total / (double(simds) * 4.0));
+
+    struct Item {
+        int64_t k;
+        int L, R, nst;
+        int64_t lt;
+    };
+    std::vector<Item> items(static_cast<size_t>(n));
+    for (int64_t k = 0; k < n; k++) {
+        int L = 64, R = Rs.back();
+        if (ls[k] > 0 && lt[k] > 0) choose_layout(ls[k], lt[k], Rs, cap, L, R);
+        else { L = 1; R = Rs.front(); }
+        const int64_t rps = int64_t(L) * R;
+        const int nst = ls[k] > 0 && lt[k] > 0 ? int((ls[k] + rps - 1) / rps) : 0;
+        items[size_t(k)] = Item{k, L, R, nst, lt[k]};
+    }
+    // group identical layouts with similar shape: sort by R, L, strips, len_t
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
+        if (a.R != b.R) return a.R < b.R;
+        if (a.L != b.L) return a.L < b.L;
+        if (a.nst != b.nst) return a.nst > b.nst;
+        return a.lt > b.lt;
+    });
+    Plan plan;
+    plan.probs.reserve(static_cast<size_t>(n));
+    struct GroupR {
+        Group g;
+        int R;
+    };
+    std::vector<GroupR> gs;
+    size_t p = 0;
+    while (p < items.size()) {
+        const int L = items[p].L, R = items[p].R;
+        const int per = 64 / L;
+        Group g{};
+        g.first = int32_t(plan.probs.size());
+        g.L = L;
+        g.lgL = ilog2(L);
+        int cnt = 0;
+        while (p < items.size() && cnt < per && items[p].L == L && items[p].R == R) {
+            Prob pr{};
+            pr.out = int32_t(items[p].k);
+            pr.L = L;
+            pr.R = R;
+            plan.probs.push_back(pr);
+            g.max_lt = std::max<int32_t>(g.max_lt, int32_t(items[p].lt));
+            g.max_nst = std::max<int32_t>(g.max_nst, items[p].nst);
+            cnt++;
+            p++;
+        }
+        g.count = cnt;
+        g.cost = int64_t(g.max_nst) * (g.max_lt + L - 1) * R;
+        gs.push_back(GroupR{g, R});
+    }
+    // one launch per R; inside, longest groups first (LPT)
+    std::stable_sort(gs.begin(), gs.end(), [](const GroupR &a, const GroupR &b) {
+        if (a.R != b.R) return a.R > b.R;
+        return a.g.cost > b.g.cost;
+    });
+    size_t q = 0;
+    while (q < gs.size()) {
+        const int R = gs[q].R;
+        const int first = int(plan.groups.size());
+        while (q < gs.size() && gs[q].R == R) plan.groups.push_back(gs[q++].g);
+        plan.launches.push_back({R, {first, int(plan.groups.size())}});
+    }
+    return plan;
+}
+
+// The plain form: the reference's own batching (ProblemBag.batches,
+// instance.py:230-275) with 64 lanes, one problem per lane (L = 1), every
+// lane padded to its batch's master shape, groups in generator order.
+Plan plan_plain(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n) {
+    const int B = 64;
+    const int R = (job.form == F_TABLE && job.tr == TR_SCORE) ? 16 : 8;
+    std::vector<uint64_t> code(static_cast<size_t>(n));
+    for (int64_t k = 0; k < n; k++) code[size_t(k)] = morton(uint64_t(ls[k]), uint64_t(lt[k]));
+    std::vector<int64_t> sorted(static_cast<size_t>(n));
+    std::iota(sorted.begin(), sorted.end(), 0);
+    std::stable_sort(sorted.begin(), sorted.end(),
+                     [&](int64_t a, int64_t b) { return code[size_t(a)] < code[size_t(b)]; });
+    std::vector<char> used(size_t(n), 0);
+    Plan plan;
+    while (true) {
+        while (!sorted.empty() && used[size_t(sorted.back())]) sorted.pop_back();
+        if (sorted.empty()) break;
+        const int64_t m = sorted.back();
+        sorted.pop_back();
+        std::vector<int64_t> idx{m};
+        const int64_t ms = ls[m], mt = lt[m];
+        for (size_t kk = 1; kk <= sorted.size(); kk++) {
+            const int64_t i = sorted[sorted.size() - kk];
+            if (used[size_t(i)]) continue;
+            if (ls[i] <= ms && lt[i] <= mt) {
+                idx.push_back(i);
+                used[size_t(i)] = 1;
+                if (int(idx.size()) == B) break;
+            }
+        }
+        Group g{};
+        g.first = int32_t(plan.probs.size());
+        g.count = int32_t(idx.size());
+        g.L = 1;
+        g.lgL = 0;
+        g.max_lt = int32_t(mt);
+        g.max_nst = int32_t((ms + R - 1) / R);
+        g.cost = int64_t(g.max_nst) * mt * R;
+        for (int64_t i : idx) {
+            Prob pr{};
+            pr.out = int32_t(i);
+            pr.L = 1;
+            pr.R = R;
+            plan.probs.push_back(pr);
+        }
+        plan.groups.push_back(g);
+    }
+    plan.launches.push_back({R, {0, int(plan.groups.size())}});
+    return plan;
+}
+
+// ---------------------------------------------------------------------------
+// a solve
+// ---------------------------------------------------------------------------
+
+struct Input {
+    int form;  // PA_FORM_*
+    bool on_device;
+    const void *codes_s, *codes_t;
+    int64_t bytes_s, bytes_t;  // payload sizes (host uploads)
+    const int64_t *off_s, *off_t;
+    const int32_t *len_s, *len_t;
+    int64_t n;
+    std::vector<float> table;  // (K+1)^2 with pad, kernel layout
+    int K1 = 1;
+    float eq = 0, ne = 0;
+    const float *dense = nullptr;
+    int64_t dense_elems = 0;
+    const int64_t *dense_off = nullptr;
+    int32_t dense_si_fixed = 0, dense_sj_fixed = 0;  // reference layout strides (0 = row-major)
+    int dim = 0;
+    int schedule = PA_SCHED_AUTO;
+    bool nan_free = true;
+};
+
+int goal_to_tr(int goal) { return goal == PA_GOAL_SCORE ? TR_SCORE : (goal == PA_GOAL_ONE ? TR_ONE : TR_ALL); }
+
+template <class T>
+void upload(hipStream_t st, DevBuf &buf, const T *src, size_t count) {
+    buf.ensure(std::max<size_t>(count * sizeof(T), 4));
+    if (count) HIPCHK(hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, float *scores_device) {
+    ensure_device(s);
+    const pa_options &o = s.opt;
+    const int64_t n = in.n;
+    for (int64_t k = 0; k < n; k++) {
+        if (in.len_s[k] < 0 || in.len_t[k] < 0) fail(PA_EINVAL, "negative sequence length");
+        if (in.len_s[k] > s.lim_s) {
+            std::ostringstream e;
+            e << "sequence s of length " << in.len_s[k] << " exceeds configured maximum length " << s.lim_s;
+            fail(PA_ELENGTH, e.str());
+        }
+        if (in.len_t[k] > s.lim_t) {
+            std::ostringstream e;
+            e << "sequence t of length " << in.len_t[k] << " exceeds configured maximum length " << s.lim_t;
+            fail(PA_ELENGTH, e.str());
+        }
+    }
+    Job job;
+    job.loc = o.type == PA_TYPE_DTW ? LOC_GLOBAL : o.loc;
+    job.dir = o.dir == PA_MAXIMIZE ? DIR_MAX : DIR_MIN;
+    job.gap = o.gap;
+    job.tr = goal_to_tr(o.goal);
+    switch (in.form) {
+        case PA_FORM_DENSE: job.form = F_DENSE; break;
+        case PA_FORM_INDEXED: job.form = F_TABLE; break;
+        case PA_FORM_BINARY: job.form = F_BINARY; break;
+        case PA_FORM_SERIES: job.form = F_SERIES; break;
+        default: fail(PA_EINVAL, "unknown problem form");
+    }
+    if (job.form == F_SERIES && o.type != PA_TYPE_DTW) fail(PA_EINVAL, "the series form is for DTW solvers");
+    if (o.type == PA_TYPE_DTW && job.form != F_DENSE && job.form != F_SERIES)
+        fail(PA_EINVAL, "DTW solvers take dense costs or series");
+    if (job.form == F_SERIES && (in.dim < 1 || in.dim > 4)) fail(PA_EINVAL, "series dimension must be 1..4");
+    job.fast = job.form == F_TABLE && job.tr == TR_SCORE && job.loc == LOC_LOCAL && job.dir == DIR_MAX &&
+               o.gap != GAP_DTW && in.nan_free;
+
+    hipStream_t st = s.stream;
+    Plan plan = in.schedule == PA_SCHED_PLAIN ? plan_plain(job, in.len_s, in.len_t, n)
+                                              : plan_auto(job, in.len_s, in.len_t, n, s.cus * 4);
+    // offsets: payload, traceback, row buffer, D values
+    const int bits = tb_bits(job.gap, job.tr);
+    for (Prob &pr : plan.probs) {
+        const int64_t k = pr.out;
+        pr.len_s = in.len_s[k];
+        pr.len_t = in.len_t[k];
+        if (in.form == PA_FORM_DENSE) {
+            pr.off_s = in.dense_off ? in.dense_off[k] : k;
+            pr.off_t = 0;
+            pr.si = in.dense_si_fixed ? in.dense_si_fixed : pr.len_t;
+            pr.sj = in.dense_sj_fixed ? in.dense_sj_fixed : 1;
+        } else {
+            pr.off_s = in.off_s[k];
+            pr.off_t = in.off_t[k];
+        }
+        const int64_t rps = int64_t(pr.L) * pr.R;
+        const int64_t nst = (pr.len_s > 0 && pr.len_t > 0) ? (pr.len_s + rps - 1) / rps : 0;
+        if (bits) {
+            pr.tb_off = plan.tb_words;
+            plan.tb_words += nst * (pr.len_t + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
+        }
+        if (nst > 1) {
+            pr.rb_off = plan.rowbuf;
+            plan.rowbuf += pr.len_t;
+        }
+        if (job.tr == TR_ALL && job.loc == LOC_LOCAL) {
+            pr.dv_off = plan.dvals;
+            plan.dvals += int64_t(pr.len_s) * pr.len_t;
+        }
+        plan.cells += double(pr.len_s) * double(pr.len_t);
+    }
+
+    // ---- device buffers ----------------------------------------------------
+    upload(st, s.b_probs, plan.probs.data(), plan.probs.size());
+    upload(st, s.b_groups, plan.groups.data(), plan.groups.size());
+    s.b_queue.ensure(std::max<size_t>(plan.launches.size(), 1) * 4);
+    HIPCHK(launch_zero_u32(s.b_queue.as<uint32_t>(), int64_t(plan.launches.size()), st));
+    const void *codes_s = in.codes_s, *codes_t = in.codes_t;
+    const float *dense = in.dense;
+    if (!in.on_device) {
+        if (in.form == PA_FORM_DENSE) {
+            upload(st, s.b_dense, in.dense, size_t(in.dense_elems));
+            dense = s.b_dense.as<float>();
+        } else {
+            upload(st, s.b_cs, static_cast<const uint8_t *>(in.codes_s), size_t(in.bytes_s));
+            upload(st, s.b_ct, static_cast<const uint8_t *>(in.codes_t), size_t(in.bytes_t));
+            codes_s = s.b_cs.p;
+            codes_t = s.b_ct.p;
+        }
+    }
+    if (job.form == F_TABLE) upload(st, s.b_table, in.table.data(), in.table.size());
+    s.b_rowbuf.ensure(size_t(std::max<int64_t>(plan.rowbuf, 1)) * sizeof(float2));
+    float *scores = scores_device;
+    if (!scores) {
+        s.b_scores.ensure(size_t(std::max<int64_t>(n, 1)) * 4);
+        scores = s.b_scores.as<float>();
+    }
+    int32_t *seeds = nullptr;
+    if (job.tr != TR_SCORE) {
+        s.b_seeds.ensure(size_t(std::max<int64_t>(n, 1)) * 8);
+        seeds = s.b_seeds.as<int32_t>();
+        s.b_tb.ensure(size_t(std::max<int64_t>(plan.tb_words, 1)) * 4);
+    }
+    if (job.tr == TR_ALL) s.b_dvals.ensure(size_t(std::max<int64_t>(plan.dvals, 1)) * 4);
+
+    KArgs a{};
+    a.probs = s.b_probs.as<Prob>();
+    a.groups = s.b_groups.as<Group>();
+    a.codes_s = codes_s;
+    a.codes_t = codes_t;
+    a.table = s.b_table.as<float>();
+    a.K1 = in.K1;
+    a.dim = in.dim;
+    a.eq = in.eq;
+    a.ne = in.ne;
+    a.dense = dense;
+    a.rowbuf = s.b_rowbuf.as<float2>();
+    a.border_s = s.b_border_s.as<float>();
+    a.border_t = s.b_border_t.as<float>();
+    a.go_s = s.go_s;
+    a.ge_s = s.ge_s;
+    a.go_t = s.go_t;
+    a.ge_t = s.ge_t;
+    a.scores = scores;
+    a.seeds = seeds;
+    a.tb = job.tr != TR_SCORE ? s.b_tb.as<uint32_t>() : nullptr;
+    a.dvals = job.tr == TR_ALL ? s.b_dvals.as<float>() : nullptr;
+
+    HIPCHK(hipEventRecord(s.ev0, st));
+    int launches = 0;
+    for (size_t li = 0; li < plan.launches.size(); li++) {
+        const int R = plan.launches[li].first;
+        const int g0 = plan.launches[li].second.first, g1 = plan.launches[li].second.second;
+        if (g1 <= g0) continue;
+        LaunchSpec spec{};
+        spec.loc = job.loc;
+        spec.dir = job.dir;
+        spec.gap = job.gap;
+        spec.form = job.form;
+        spec.tr = job.tr;
+        spec.R = R;
+        spec.fastmax = job.fast;
+        spec.lds_bytes = job.form == F_TABLE ? size_t(in.K1) * in.K1 * 4 : 0;
+        const int waves_per_cu = std::max(1, occupancy_waves(spec));
+        const int64_t wg_needed = (int64_t(g1 - g0) + 3) / 4;
+        spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg_needed, int64_t(s.cus) * std::max(1, waves_per_cu / 4))));
+        KArgs la = a;
+        la.groups = a.groups + g0;
+        la.n_groups = g1 - g0;
+        la.queue = s.b_queue.as<uint32_t>() + li;
+        const hipError_t e = launch_solve(spec, la, st);
+        if (e != hipSuccess) {
+            std::ostringstream m;
+            m << "kernel launch failed (loc " << spec.loc << " dir " << spec.dir << " gap " << spec.gap << " form "
+              << spec.form << " tr " << spec.tr << " R " << spec.R << "): " << hipGetErrorString(e);
+            fail(PA_EHIP, m.str());
+        }
+        launches++;
+    }
+    HIPCHK(hipEventRecord(s.ev1, st));
+    s.last_launches = launches;
+    s.last_cells = plan.cells;
+
+    if (!result) {
+        return;  // device scores: caller synchronises
+    }
+    auto r = std::make_unique<pa_result>();
+    r->n = n;
+    r->dir = o.dir;
+    r->loc = job.loc;
+    r->gap = job.gap;
+    r->goal = o.goal;
+    r->index_bits = o.index_bits;
+    r->remove_dup = o.remove_dup;
+    r->len_s.assign(in.len_s, in.len_s + n);
+    r->len_t.assign(in.len_t, in.len_t + n);
+    r->scores.resize(static_cast<size_t>(n));
+    if (job.tr == TR_ONE) {
+        r->path_off.resize(size_t(n) + 1);
+        r->path_off[0] = 0;
+        for (int64_t k = 0; k < n; k++) r->path_off[size_t(k) + 1] = r->path_off[size_t(k)] + in.len_s[k] + in.len_t[k] + 2;
+        upload(st, s.b_path_off, r->path_off.data(), r->path_off.size());
+        s.b_path.ensure(size_t(std::max<int64_t>(r->path_off.back(), 1)) * 8);
+        s.b_path_len.ensure(size_t(std::max<int64_t>(n, 1)) * 4);
+        HIPCHK(launch_walk(job.gap, s.b_probs.as<Prob>(), int64_t(plan.probs.size()), s.b_tb.as<uint32_t>(), seeds,
+                           s.b_path.as<int32_t>(), s.b_path_off.as<int64_t>(), s.b_path_len.as<int32_t>(), st));
+        r->path_len.resize(static_cast<size_t>(n));
+        r->paths.resize(size_t(r->path_off.back()) * 2);
+        HIPCHK(hipMemcpyAsync(r->path_len.data(), s.b_path_len.p, size_t(n) * 4, hipMemcpyDeviceToHost, st));
+        if (!r->paths.empty())
+            HIPCHK(hipMemcpyAsync(r->paths.data(), s.b_path.p, r->paths.size() * 4, hipMemcpyDeviceToHost, st));
+    }
+    if (job.tr != TR_SCORE) {
+        r->seeds.resize(size_t(n) * 2);
+        HIPCHK(hipMemcpyAsync(r->seeds.data(), seeds, size_t(n) * 8, hipMemcpyDeviceToHost, st));
+    }
+    if (job.tr == TR_ALL) {
+        r->tb.resize(size_t(plan.tb_words));
+        if (!r->tb.empty())
+            HIPCHK(hipMemcpyAsync(r->tb.data(), s.b_tb.p, r->tb.size() * 4, hipMemcpyDeviceToHost, st));
+        r->dvals.resize(size_t(plan.dvals));
+        if (!r->dvals.empty())
+            HIPCHK(hipMemcpyAsync(r->dvals.data(), s.b_dvals.p, r->dvals.size() * 4, hipMemcpyDeviceToHost, st));
+        r->probs.resize(static_cast<size_t>(n));
+        for (const Prob &pr : plan.probs) r->probs[size_t(pr.out)] = pr;
+        r->iters.resize(static_cast<size_t>(n));
+    }
+    if (n) HIPCHK(hipMemcpyAsync(r->scores.data(), scores, size_t(n) * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    s.last_ms = ms;
+    *result = std::move(r);
+}
+
+// kernel table layout: (K+1) x (K+1), row = s symbol, last row/column = pad
+// symbol with the worst value so padded rows never win
+std::vector<float> make_table(const float *sim, int64_t K0, int64_t K1, int dir, bool *nan_free) {
+    const int64_t K = std::max(K0, K1);
+    const int64_t S = K + 1;
+    std::vector<float> t(size_t(S * S), worst(dir));
+    bool ok = true;
+    for (int64_t i = 0; i < K0; i++)
+        for (int64_t j = 0; j < K1; j++) {
+            const float x = sim[i * K1 + j];
+            if (std::isnan(x)) ok = false;
+            t[size_t(i * S + j)] = x;
+        }
+    *nan_free = ok;
+    return t;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char *pa_last_error(void) { return g_err.c_str(); }
+
+int pa_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pa_create_options(const pa_options_desc *d, pa_options **out) {
+    return guard([&] {
+        if (!d || !out) fail(PA_EINVAL, "null argument");
+        auto o = std::make_unique<pa_options>();
+        o->d = *d;
+        o->type = d->solver_type;
+        o->dir = d->direction;
+        o->remove_dup = !d->return_dup;
+        o->index_bits = d->index_bits;
+        if (o->type != PA_TYPE_ALIGNMENT && o->type != PA_TYPE_DTW) fail(PA_EINVAL, "illegal solver type");
+        if (o->dir != PA_MINIMIZE && o->dir != PA_MAXIMIZE) fail(PA_EINVAL, "illegal direction");
+        if (o->index_bits != 16 && o->index_bits != 32) fail(PA_EINVAL, "index bits must be 16 or 32");
+        if (o->type == PA_TYPE_DTW) {
+            // DTW solvers are always one-optimal (factory.h:1072-1099)
+            o->loc = PA_GLOBAL;
+            o->goal = PA_GOAL_ONE;
+            o->gap = GAP_DTW;
+            *out = o.release();
+            return;
+        }
+        o->loc = d->locality;
+        if (o->loc != PA_LOCAL && o->loc != PA_GLOBAL && o->loc != PA_SEMIGLOBAL) fail(PA_EINVAL, "invalid locality");
+        if (d->count == PA_COUNT_ONE) {
+            if (d->detail == PA_DETAIL_SCORE) o->goal = PA_GOAL_SCORE;
+            else if (d->detail == PA_DETAIL_ALIGNMENT || d->detail == PA_DETAIL_SOLUTION) o->goal = PA_GOAL_ONE;
+            else fail(PA_EINVAL, "invalid detail");
+        } else if (d->count == PA_COUNT_ALL) {
+            if (d->detail < PA_DETAIL_SCORE || d->detail > PA_DETAIL_SOLUTION) fail(PA_EINVAL, "invalid detail");
+            o->goal = PA_GOAL_ALL;
+        } else {
+            fail(PA_EINVAL, "invalid count");
+        }
+        auto kind = [](int k) { return k == PA_GAP_NONE ? PA_GAP_LINEAR : k; };
+        const int ks = kind(d->gap_kind_s), kt = kind(d->gap_kind_t);
+        if (ks == PA_GAP_LINEAR && kt == PA_GAP_LINEAR) {
+            o->gap = GAP_LIN;
+            o->lin_s = d->gap_kind_s == PA_GAP_NONE ? 0.0f : float(d->gap_s[0]);
+            o->lin_t = d->gap_kind_t == PA_GAP_NONE ? 0.0f : float(d->gap_t[0]);
+            if (o->lin_s < 0 || o->lin_t < 0) fail(PA_EINVAL, "gap costs must be non-negative");
+        } else if (ks == PA_GAP_AFFINE && kt == PA_GAP_AFFINE) {
+            o->gap = GAP_AFF;
+            // flip (u, v) of pyalign.gaps into AffineCost(u = v, v = u) (factory.h:40-52)
+            o->aff_u_s = float(d->gap_s[1]);
+            o->aff_v_s = float(d->gap_s[0]);
+            o->aff_u_t = float(d->gap_t[1]);
+            o->aff_v_t = float(d->gap_t[0]);
+            if (o->aff_u_s < 0 || o->aff_v_s < 0 || o->aff_u_t < 0 || o->aff_v_t < 0)
+                fail(PA_EINVAL, "gap costs must be non-negative");
+        } else {
+            fail(PA_EINVAL,
+                 "general gap costs (Waterman-Smith-Beyer) are not supported by the MI355X solver; use "
+                 "LinearGapCost or AffineGapCost");
+        }
+        *out = o.release();
+    });
+}
+
+void pa_options_free(pa_options *o) { delete o; }
+
+int pa_create_solver(int64_t max_len_s, int64_t max_len_t, const pa_options *opts, pa_solver **out) {
+    return guard([&] {
+        if (!opts || !out) fail(PA_EINVAL, "null argument");
+        const int64_t imax = (opts->index_bits == 16 ? int64_t(INT16_MAX) : int64_t(INT32_MAX)) >> 1;
+        auto check = [&](const char *name, int64_t len) {  // solver.h:496-503
+            if (len > imax) {
+                std::ostringstream e;
+                e << "requested maximum length " << len << " for " << name
+                  << " exceeds maximum supported sequence length in this implementation " << imax;
+                fail(PA_ELENGTH, e.str());
+            }
+        };
+        check("s", max_len_s);
+        check("t", max_len_t);
+        if (max_len_s < 0 || max_len_t < 0) fail(PA_EINVAL, "negative maximum length");
+        auto s = std::make_unique<pa_solver>();
+        s->opt = *opts;
+        s->lim_s = max_len_s;
+        s->lim_t = max_len_t;
+        make_borders(*s);
+        const float sgn = opts->dir == PA_MINIMIZE ? 1.0f : -1.0f;
+        if (opts->gap == GAP_LIN) {
+            s->go_s = opts->lin_s * sgn;
+            s->go_t = opts->lin_t * sgn;
+        } else if (opts->gap == GAP_AFF) {
+            s->go_s = (opts->aff_u_s + opts->aff_v_s) * sgn;  // w1 = u + v (solver.h:2872-2874)
+            s->ge_s = opts->aff_u_s * sgn;
+            s->go_t = (opts->aff_u_t + opts->aff_v_t) * sgn;
+            s->ge_t = opts->aff_u_t * sgn;
+        }
+        *out = s.release();
+    });
+}
+
+void pa_solver_free(pa_solver *s) { delete s; }
+
+int pa_solver_batch_size(const pa_solver *s) {
+    // batch=True solvers take one wavefront of lanes per call; unbatched take 1
+    return s && s->opt.d.batch ? 64 : 1;
+}
+
+static void check_batch(const pa_solver *s, int64_t B) {  // factory.h:387-394
+    const int bs = pa_solver_batch_size(s);
+    if (B != bs) {
+        std::ostringstream e;
+        e << "problem of batch size " << B << " given to solver that was configured to batch size " << bs;
+        fail(PA_EINVAL, e.str());
+    }
+}
+
+static void lanes_from_len(const int32_t *len, int64_t B, int64_t Ls, int64_t Lt, std::vector<int32_t> &ls,
+                           std::vector<int32_t> &lt) {
+    ls.assign(len, len + B);
+    lt.assign(len + B, len + 2 * B);
+    for (int64_t k = 0; k < B; k++)
+        if (ls[size_t(k)] > Ls || lt[size_t(k)] > Lt || ls[size_t(k)] < 0 || lt[size_t(k)] < 0)
+            fail(PA_EINVAL, "problem length exceeds the batch's matrix shape");
+}
+
+static void check_master(const pa_solver *s, int64_t Ls, int64_t Lt) {  // solver.h:725-726
+    if (Ls > s->lim_s) {
+        std::ostringstream e;
+        e << "sequence s of length " << Ls << " exceeds configured maximum length " << s->lim_s;
+        fail(PA_ELENGTH, e.str());
+    }
+    if (Lt > s->lim_t) {
+        std::ostringstream e;
+        e << "sequence t of length " << Lt << " exceeds configured maximum length " << s->lim_t;
+        fail(PA_ELENGTH, e.str());
+    }
+}
+
+int pa_solve_dense(pa_solver *s, const float *sim, int64_t Ls, int64_t Lt, int64_t B, const int32_t *len,
+                   pa_result **out) {
+    return guard([&] {
+        if (!s || !len || !out || (!sim && Ls * Lt * B > 0)) fail(PA_EINVAL, "null argument");
+        check_batch(s, B);
+        check_master(s, Ls, Lt);
+        std::vector<int32_t> ls, lt;
+        lanes_from_len(len, B, Ls, Lt, ls, lt);
+        Input in{};
+        in.form = PA_FORM_DENSE;
+        in.n = B;
+        in.len_s = ls.data();
+        in.len_t = lt.data();
+        in.dense = sim;
+        in.dense_elems = Ls * Lt * B;
+        in.dense_si_fixed = int32_t(Lt * B);  // sim[i, j, k] at (i*Lt + j)*B + k
+        in.dense_sj_fixed = int32_t(B);
+        std::unique_ptr<pa_result> r;
+        run(*s, in, &r, nullptr);
+        *out = r.release();
+    });
+}
+
+static void indexed_input(Input &in, const uint32_t *a, int64_t La, const uint32_t *b, int64_t Lb, int64_t B,
+                          const float *sim, int64_t K0, int64_t K1, int dir, std::vector<uint8_t> &cs,
+                          std::vector<uint8_t> &ct, std::vector<int64_t> &os, std::vector<int64_t> &ot) {
+    uint32_t amax = 0, bmax = 0;
+    for (int64_t k = 0; k < B * La; k++) amax = std::max(amax, a[k]);
+    for (int64_t k = 0; k < B * Lb; k++) bmax = std::max(bmax, b[k]);
+    // indexed_matrix_form::check (factory.h:602-614)
+    if (B * La > 0 && amax >= uint64_t(K0)) fail(PA_EINVAL, "out of bounds index in a");
+    if (B * Lb > 0 && bmax >= uint64_t(K1)) fail(PA_EINVAL, "out of bounds index in b");
+    if (std::max(K0, K1) + 1 > MAX_LDS_TABLE)
+        fail(PA_EINVAL, "alphabets above 128 symbols are not supported by the MI355X indexed form");
+    cs.resize(size_t(B * La) + 1);
+    ct.resize(size_t(B * Lb) + 1);
+    for (int64_t k = 0; k < B * La; k++) cs[size_t(k)] = uint8_t(a[k]);
+    for (int64_t k = 0; k < B * Lb; k++) ct[size_t(k)] = uint8_t(b[k]);
+    os.resize(static_cast<size_t>(B));
+    ot.resize(static_cast<size_t>(B));
+    for (int64_t k = 0; k < B; k++) {
+        os[size_t(k)] = k * La;
+        ot[size_t(k)] = k * Lb;
+    }
+    in.form = PA_FORM_INDEXED;
+    in.codes_s = cs.data();
+    in.codes_t = ct.data();
+    in.bytes_s = int64_t(cs.size());
+    in.bytes_t = int64_t(ct.size());
+    in.off_s = os.data();
+    in.off_t = ot.data();
+    bool nan_free = true;
+    in.table = make_table(sim, K0, K1, dir, &nan_free);
+    in.nan_free = nan_free;
+    in.K1 = int(std::max(K0, K1) + 1);
+}
+
+int pa_solve_indexed(pa_solver *s, const uint32_t *a, int64_t La, const uint32_t *b, int64_t Lb, int64_t B,
+                     const float *sim, int64_t K0, int64_t K1, const int32_t *len, pa_result **out) {
+    return guard([&] {
+        if (!s || !len || !out || !sim) fail(PA_EINVAL, "null argument");
+        check_batch(s, B);
+        check_master(s, La, Lb);
+        std::vector<int32_t> ls, lt;
+        lanes_from_len(len, B, La, Lb, ls, lt);
+        Input in{};
+        std::vector<uint8_t> cs, ct;
+        std::vector<int64_t> os, ot;
+        indexed_input(in, a, La, b, Lb, B, sim, K0, K1, s->opt.dir, cs, ct, os, ot);
+        in.n = B;
+        in.len_s = ls.data();
+        in.len_t = lt.data();
+        std::unique_ptr<pa_result> r;
+        run(*s, in, &r, nullptr);
+        *out = r.release();
+    });
+}
+
+int pa_solve_binary(pa_solver *s, const uint32_t *a, int64_t La, const uint32_t *b, int64_t Lb, int64_t B,
+                    float eq, float ne, const int32_t *len, pa_result **out) {
+    return guard([&] {
+        if (!s || !len || !out) fail(PA_EINVAL, "null argument");
+        check_batch(s, B);
+        check_master(s, La, Lb);
+        std::vector<int32_t> ls, lt;
+        lanes_from_len(len, B, La, Lb, ls, lt);
+        std::vector<int64_t> os(static_cast<size_t>(B)), ot(static_cast<size_t>(B));
+        for (int64_t k = 0; k < B; k++) {
+            os[size_t(k)] = k * La;
+            ot[size_t(k)] = k * Lb;
+        }
+        static const uint32_t zero = 0;
+        Input in{};
+        in.form = PA_FORM_BINARY;
+        in.n = B;
+        in.len_s = ls.data();
+        in.len_t = lt.data();
+        in.codes_s = B * La ? static_cast<const void *>(a) : &zero;
+        in.codes_t = B * Lb ? static_cast<const void *>(b) : &zero;
+        in.bytes_s = std::max<int64_t>(B * La, 1) * 4;
+        in.bytes_t = std::max<int64_t>(B * Lb, 1) * 4;
+        in.off_s = os.data();
+        in.off_t = ot.data();
+        in.eq = eq;
+        in.ne = ne;
+        std::unique_ptr<pa_result> r;
+        run(*s, in, &r, nullptr);
+        *out = r.release();
+    });
+}
+
+static void ragged_input(const pa_solver *s, const pa_ragged *b, Input &in) {
+    if (!b) fail(PA_EINVAL, "null batch");
+    in.n = b->n;
+    in.len_s = b->len_s;
+    in.len_t = b->len_t;
+    in.on_device = b->on_device != 0;
+    in.off_s = b->off_s;
+    in.off_t = b->off_t;
+    in.form = b->form;
+    int64_t end_s = 0, end_t = 0;
+    if (b->form != PA_FORM_DENSE) {
+        if (!b->off_s || !b->off_t) fail(PA_EINVAL, "null offsets");
+        for (int64_t k = 0; k < b->n; k++) {
+            end_s = std::max<int64_t>(end_s, b->off_s[k] + b->len_s[k]);
+            end_t = std::max<int64_t>(end_t, b->off_t[k] + b->len_t[k]);
+        }
+    }
+    switch (b->form) {
+        case PA_FORM_INDEXED: {
+            if (b->K < 1 || b->K + 1 > MAX_LDS_TABLE) fail(PA_EINVAL, "alphabet size must be 1..128");
+            if (!b->table) fail(PA_EINVAL, "null table");
+            in.bytes_s = end_s;
+            in.bytes_t = end_t;
+            if (!b->on_device) {
+                // every code indexes the table staged in LDS
+                const uint8_t *cs = static_cast<const uint8_t *>(b->codes_s);
+                const uint8_t *ct = static_cast<const uint8_t *>(b->codes_t);
+                for (int64_t k = 0; k < b->n; k++) {
+                    for (int64_t i = 0; i < b->len_s[k]; i++)
+                        if (cs[b->off_s[k] + i] >= b->K) fail(PA_EINVAL, "out of bounds index in a");
+                    for (int64_t j = 0; j < b->len_t[k]; j++)
+                        if (ct[b->off_t[k] + j] >= b->K) fail(PA_EINVAL, "out of bounds index in b");
+                }
+            }
+            bool nan_free = true;
+            in.table = make_table(b->table, b->K, b->K, s->opt.dir, &nan_free);
+            in.nan_free = nan_free;
+            in.K1 = b->K + 1;
+            break;
+        }
+        case PA_FORM_BINARY:
+            in.bytes_s = end_s * 4;
+            in.bytes_t = end_t * 4;
+            in.eq = b->eq;
+            in.ne = b->ne;
+            break;
+        case PA_FORM_SERIES:
+            in.dim = b->dim;
+            in.bytes_s = end_s * 4 * b->dim;
+            in.bytes_t = end_t * 4 * b->dim;
+            break;
+        case PA_FORM_DENSE: {
+            if (!b->dense_off) fail(PA_EINVAL, "null dense offsets");
+            int64_t end = 0;
+            for (int64_t k = 0; k < b->n; k++) end = std::max<int64_t>(end, b->dense_off[k] + int64_t(b->len_s[k]) * b->len_t[k]);
+            in.dense = b->dense;
+            in.dense_elems = end;
+            in.dense_off = b->dense_off;
+            break;
+        }
+        default: fail(PA_EINVAL, "unknown problem form");
+    }
+    in.codes_s = b->codes_s;
+    in.codes_t = b->codes_t;
+    static const uint32_t zero = 0;
+    if (!in.codes_s) in.codes_s = &zero;
+    if (!in.codes_t) in.codes_t = &zero;
+    in.bytes_s = std::max<int64_t>(in.bytes_s, 1);
+    in.bytes_t = std::max<int64_t>(in.bytes_t, 1);
+}
+
+int pa_solve_ragged(pa_solver *s, const pa_ragged *batch, int32_t schedule, pa_result **out) {
+    return guard([&] {
+        if (!s || !out) fail(PA_EINVAL, "null argument");
+        Input in{};
+        ragged_input(s, batch, in);
+        in.schedule = schedule;
+        std::unique_ptr<pa_result> r;
+        run(*s, in, &r, nullptr);
+        *out = r.release();
+    });
+}
+
+int pa_solve_ragged_scores_device(pa_solver *s, const pa_ragged *batch, int32_t schedule, float *scores_device) {
+    return guard([&] {
+        if (!s || !scores_device) fail(PA_EINVAL, "null argument");
+        if (s->opt.goal != PA_GOAL_SCORE && s->opt.type != PA_TYPE_DTW)
+            fail(PA_EINVAL, "device score output needs a score-only solver");
+        Input in{};
+        ragged_input(s, batch, in);
+        in.schedule = schedule;
+        run(*s, in, nullptr, scores_device);
+    });
+}
+
+int pa_solver_sync(pa_solver *s) {
+    return guard([&] {
+        if (!s) fail(PA_EINVAL, "null solver");
+        if (!s->stream) return;
+        HIPCHK(hipStreamSynchronize(s->stream));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->last_ms = ms;
+    });
+}
+
+int pa_solver_last_kernel_ms(const pa_solver *s, double *ms, int32_t *launches, double *cells) {
+    return guard([&] {
+        if (!s) fail(PA_EINVAL, "null solver");
+        if (ms) *ms = s->last_ms;
+        if (launches) *launches = s->last_launches;
+        if (cells) *cells = s->last_cells;
+    });
+}
+
+// ---- results ---------------------------------------------------------------
+
+int64_t pa_result_count(const pa_result *r) { return r ? r->n : 0; }
+
+int pa_result_scores(const pa_result *r, float *out) {
+    return guard([&] {
+        if (!r || !out) fail(PA_EINVAL, "null argument");
+        std::copy(r->scores.begin(), r->scores.end(), out);
+    });
+}
+
+int pa_result_has_path(const pa_result *r, int64_t k) {
+    if (!r || k < 0 || k >= r->n) return 0;
+    if (r->goal == PA_GOAL_ONE) return r->path_len[size_t(k)] >= 0 ? 1 : 0;
+    return 0;
+}
+
+int64_t pa_result_path_len(const pa_result *r, int64_t k) {
+    if (!r || k < 0 || k >= r->n || r->goal != PA_GOAL_ONE) return -1;
+    return r->path_len[size_t(k)];
+}
+
+int pa_result_path(const pa_result *r, int64_t k, int32_t *uv) {
+    return guard([&] {
+        if (!r || !uv || k < 0 || k >= r->n || r->goal != PA_GOAL_ONE) fail(PA_EINVAL, "no path for this problem");
+        const int32_t m = r->path_len[size_t(k)];
+        if (m < 0) fail(PA_EINVAL, "no path for this problem");
+        std::memcpy(uv, r->paths.data() + 2 * r->path_off[size_t(k)], size_t(m) * 8);
+    });
+}
+
+// path_compressor (solver.h:809-846) + Alignment::add_edge (factory.h:164-170)
+static void alignment_from_path(const int32_t *uv, int64_t m, int32_t ls, int32_t lt, int32_t *s_to_t,
+                                int32_t *t_to_s) {
+    std::fill(s_to_t, s_to_t + ls, -1);
+    std::fill(t_to_s, t_to_s + lt, -1);
+    const bool by_s = ls <= lt;
+    std::vector<int32_t> primary(size_t(by_s ? ls : lt), -1);
+    for (int64_t q = 1; q < m; q++) {
+        const int32_t lu = uv[2 * (q - 1)], lv = uv[2 * (q - 1) + 1];
+        const int32_t u = uv[2 * q], v = uv[2 * q + 1];
+        if (u != lu && v != lv && lu >= 0 && lv >= 0) {
+            if (by_s) primary[size_t(lu)] = lv;
+            else primary[size_t(lv)] = lu;
+        }
+    }
+    // the other direction comes from inversion (factory.h:110-128)
+    if (by_s) {
+        for (int32_t i = 0; i < ls; i++) s_to_t[i] = primary[size_t(i)];
+        for (int32_t i = 0; i < ls; i++)
+            if (s_to_t[i] >= 0) t_to_s[s_to_t[i]] = i;
+    } else {
+        for (int32_t j = 0; j < lt; j++) t_to_s[j] = primary[size_t(j)];
+        for (int32_t j = 0; j < lt; j++)
+            if (t_to_s[j] >= 0) s_to_t[t_to_s[j]] = j;
+    }
+}
+
+int pa_result_alignment(const pa_result *r, int64_t k, int32_t *s_to_t, int32_t *t_to_s) {
+    return guard([&] {
+        if (!r || k < 0 || k >= r->n) fail(PA_EINVAL, "bad problem index");
+        const int32_t ls = r->len_s[size_t(k)], lt = r->len_t[size_t(k)];
+        if (r->goal == PA_GOAL_ONE) {
+            const int32_t m = r->path_len[size_t(k)];
+            if (m < 0) {
+                std::fill(s_to_t, s_to_t + ls, -1);
+                std::fill(t_to_s, t_to_s + lt, -1);
+                return;
+            }
+            alignment_from_path(r->paths.data() + 2 * r->path_off[size_t(k)], m, ls, lt, s_to_t, t_to_s);
+        } else if (r->goal == PA_GOAL_ALL) {
+            const IterState *it = r->iters[size_t(k)].get();
+            if (!it || it->cur_path.empty()) fail(PA_EINVAL, "no current alignment");
+            std::vector<int32_t> uv;
+            for (auto &p : it->cur_path) {
+                uv.push_back(p.first);
+                uv.push_back(p.second);
+            }
+            alignment_from_path(uv.data(), int64_t(it->cur_path.size()), ls, lt, s_to_t, t_to_s);
+        } else {
+            fail(PA_EINVAL, "score-only results carry no alignment");
+        }
+    });
+}
+
+// ---- all-optimal enumeration over the GPU's 8-bit tie sets ----------------
+
+static uint32_t tie_bits(const pa_result *r, const Prob &pr, int32_t i, int32_t j) {
+    const int L = pr.L, R = pr.R;
+    const int rps = L * R;
+    const int wpc = (8 * R + 31) / 32;
+    const int st = i / rps;
+    const int rem = i - st * rps;
+    const int kk = rem / R;
+    const int rr = rem - kk * R;
+    const int64_t s = j + kk;
+    const int64_t nsteps = pr.len_t + L - 1;
+    const uint32_t w = r->tb[size_t(pr.tb_off + ((int64_t(st) * nsteps + s) * L + kk) * wpc + (rr * 8) / 32)];
+    return (w >> ((rr * 8) % 32)) & 0xffu;
+}
+
+// the edge list of a traceback_n cell: candidates equal to the cell's value
+// in push order (solver.h:1324-1427)
+static void push_gap_list(const pa_result *r, const Prob &pr, int32_t i, int32_t j, bool vertical,
+                          std::vector<std::pair<int32_t, int32_t>> &out) {
+    while (true) {
+        if (i < 0 || j < 0) return;  // border P/Q traceback is cleared (solver.h:2953-2966)
+        const uint32_t b = tie_bits(r, pr, i, j);
+        const bool open = vertical ? (b & 16u) : (b & 64u);
+        const bool ext = vertical ? (b & 32u) : (b & 128u);
+        if (open) out.emplace_back(vertical ? i - 1 : i, vertical ? j : j - 1);
+        if (!ext) return;
+        if (vertical) i -= 1;
+        else j -= 1;
+    }
+}
+
+static void cell_edges(const pa_result *r, const Prob &pr, int32_t u, int32_t v, int32_t no_tb,
+                       std::vector<std::pair<int32_t, int32_t>> &out) {
+    out.clear();
+    const uint32_t b = tie_bits(r, pr, u, v);
+    if (b & 1u) out.emplace_back(no_tb, no_tb);
+    if (b & 2u) out.emplace_back(u - 1, v - 1);
+    if (r->gap == GAP_AFF) {
+        if (b & 4u) push_gap_list(r, pr, u, v, true, out);
+        if (b & 8u) push_gap_list(r, pr, u, v, false, out);
+    } else {
+        if (b & 4u) out.emplace_back(u - 1, v);
+        if (b & 8u) out.emplace_back(u, v - 1);
+    }
+}
+
+int pa_result_iter_next(pa_result *r, int64_t k, float *score) {
+    int found = 0;
+    const int status = guard([&] {
+        if (!r || k < 0 || k >= r->n) fail(PA_EINVAL, "bad problem index");
+        if (r->goal != PA_GOAL_ALL) fail(PA_EINVAL, "not an all-optimal result");
+        auto &slot = r->iters[size_t(k)];
+        if (!slot) slot = std::make_unique<IterState>();
+        IterState &it = *slot;
+        const Prob &pr = r->probs[size_t(k)];
+        const int32_t ls = r->len_s[size_t(k)], lt = r->len_t[size_t(k)];
+        const int32_t no_tb = r->index_bits == 16 ? int32_t(INT16_MIN) : INT32_MIN;
+        const bool local = r->loc == PA_LOCAL;
+        const bool mx = r->dir == PA_MAXIMIZE;
+        auto Dval = [&](int32_t i, int32_t j) { return r->dvals[size_t(pr.dv_off + int64_t(i) * lt + j)]; };
+        auto cont = [&](int32_t u, int32_t v) {
+            if (!(u >= 0 && v >= 0)) return false;
+            if (local) return (tie_bits(r, pr, u, v) & 1u) == 0;  // D(u, v) is not zero
+            return true;
+        };
+        if (!it.started) {
+            it.started = true;
+            // seeds (solver.h:2060-2092, 2179-2187, 2273-2293)
+            if (local) {
+                if (ls > 0 && lt > 0) {
+                    float best = 0.0f;
+                    for (int32_t i = ls - 1; i >= 0; i--)
+                        for (int32_t j = lt - 1; j >= 0; j--) {
+                            const float x = Dval(i, j);
+                            if (mx ? x > best : x < best) best = x;
+                        }
+                    for (int32_t i = ls - 1; i >= 0; i--)
+                        for (int32_t j = lt - 1; j >= 0; j--) {
+                            const float x = Dval(i, j);
+                            if (x == best && (mx ? x > 0.0f : x < 0.0f))
+                                it.stack.push_back(IterState::Entry{x, i, j, 0});
+                        }
+                }
+            } else {
+                const int32_t si = r->seeds[size_t(2 * k)], sj = r->seeds[size_t(2 * k + 1)];
+                if (si != NO_SEED) {
+                    const float val = r->loc == PA_GLOBAL ? r->scores[size_t(k)] : r->scores[size_t(k)];
+                    it.stack.push_back(IterState::Entry{val, si, sj, 0});
+                }
+            }
+        }
+        std::vector<std::pair<int32_t, int32_t>> edges;
+        while (!it.stack.empty()) {
+            const IterState::Entry e = it.stack.back();
+            it.stack.pop_back();
+            if (e.path_len == 0) it.path.clear();
+            else it.path.resize(size_t(e.path_len));
+            it.path.emplace_back(e.u, e.v);
+            if (cont(e.u, e.v)) {
+                cell_edges(r, pr, e.u, e.v, no_tb, edges);
+                const int64_t ps = int64_t(it.path.size());
+                if (!edges.empty()) {
+                    for (auto &q : edges) it.stack.push_back(IterState::Entry{e.val, q.first, q.second, ps});
+                } else {
+                    it.stack.push_back(IterState::Entry{e.val, no_tb, no_tb, ps});
+                }
+            } else {
+                if (r->remove_dup) {
+                    std::vector<std::pair<int32_t, int32_t>> key;
+                    for (size_t q = 1; q < it.path.size(); q++) {
+                        const auto &a0 = it.path[q - 1], &a1 = it.path[q];
+                        if (a1.first != a0.first && a1.second != a0.second && a0.first >= 0 && a0.second >= 0)
+                            key.push_back(a0);
+                    }
+                    if (!it.emitted.insert(key).second) continue;
+                }
+                it.cur_path = it.path;
+                if (score) *score = e.val;
+                found = 1;
+                return;
+            }
+        }
+        it.cur_path.clear();
+    });
+    return status == PA_OK ? found : -status;
+}
+
+int64_t pa_result_iter_path_len(const pa_result *r, int64_t k) {
+    if (!r || k < 0 || k >= r->n || r->goal != PA_GOAL_ALL || !r->iters[size_t(k)]) return -1;
+    return int64_t(r->iters[size_t(k)]->cur_path.size());
+}
+
+int pa_result_iter_path(const pa_result *r, int64_t k, int32_t *uv) {
+    return guard([&] {
+        if (!r || !uv || k < 0 || k >= r->n || r->goal != PA_GOAL_ALL || !r->iters[size_t(k)])
+            fail(PA_EINVAL, "no current alignment");
+        const auto &p = r->iters[size_t(k)]->cur_path;
+        for (size_t q = 0; q < p.size(); q++) {
+            uv[2 * q] = p[q].first;
+            uv[2 * q + 1] = p[q].second;
+        }
+    });
+}
+
+void pa_result_free(pa_result *r) { delete r; }
+
+}  // extern "C"

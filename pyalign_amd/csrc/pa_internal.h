@@ -1,0 +1,88 @@
+// pa_internal.h -- data layouts shared by the HIP kernels and the host library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pa {
+
+enum { LOC_LOCAL = 0, LOC_GLOBAL = 1, LOC_SEMI = 2 };
+enum { DIR_MIN = 0, DIR_MAX = 1 };
+enum { GAP_LIN = 0, GAP_AFF = 1, GAP_DTW = 2 };
+// Similarity sources.  F_TABLE reads a (K+1)x(K+1) table (row = s symbol,
+// last symbol = pad whose entries are the worst value) staged in LDS with
+// uint8 codes; F_TABLE_G reads it from global memory with uint16 codes.
+enum { F_DENSE = 0, F_TABLE = 1, F_BINARY = 2, F_SERIES = 3, F_TABLE_G = 4 };
+// What the forward pass keeps: score only, one traceback direction per cell
+// (2 bits linear/DTW, 4 bits affine) or the full tie set (8 bits per cell).
+enum { TR_SCORE = 0, TR_ONE = 1, TR_ALL = 2 };
+
+constexpr int WAVE = 64;
+constexpr int MAX_LDS_TABLE = 129;  // K + 1 pad symbol
+constexpr int32_t NO_SEED = INT32_MIN;
+
+// One problem.  Offsets are element offsets into the form's payload.
+struct Prob {
+    int64_t off_s;   // codes / series rows of s, or dense base
+    int64_t off_t;   // codes / series rows of t
+    int64_t tb_off;  // first 32-bit word of this problem's traceback
+    int64_t rb_off;  // float2 offset of this problem's strip-boundary row buffer
+    int64_t dv_off;  // TR_ALL: offset of this problem's D values (row-major)
+    int32_t len_s, len_t;
+    int32_t si, sj;  // dense strides
+    int32_t out;     // output slot (caller's problem index)
+    int32_t L;       // lanes cooperating on this problem
+    int32_t R;       // rows per lane
+    int32_t pad;
+};
+
+// One wavefront's work: 64/L problems (segment k <- probs[first + k]).
+struct Group {
+    int32_t first, count;
+    int32_t L, lgL;      // lanes per problem and its log2
+    int32_t max_lt;      // over members: step loop bound is max_lt + L - 1
+    int32_t max_nst;     // over members: strips of L*R rows
+    int64_t cost;        // scheduling cost (lane-cells)
+};
+
+struct KArgs {
+    const Prob *probs;
+    const Group *groups;
+    int32_t n_groups;
+    uint32_t *queue;  // work-queue ticket counter (zeroed before launch)
+    const void *codes_s;
+    const void *codes_t;
+    const float *table;
+    int32_t K1;       // table stride = K + 1
+    int32_t dim;
+    float eq, ne;
+    const float *dense;
+    float2 *rowbuf;
+    const float *border_s;  // border_s[k] = D(k-1, -1), k = 0..lim_s
+    const float *border_t;  // border_t[k] = D(-1, k-1), k = 0..lim_t
+    float go_s, ge_s, go_t, ge_t;  // signed penalties: w1*sgn, extend*sgn (linear: g*sgn in go)
+    float *scores;
+    int32_t *seeds;    // [2*out]: seed (i, j), i = NO_SEED when there is none
+    uint32_t *tb;      // traceback words
+    float *dvals;      // TR_ALL: D values per cell
+};
+
+struct LaunchSpec {
+    int loc, dir, gap, form, tr, R;
+    bool fastmax;   // v_max_f32 is exact (local maximize score, NaN-free table)
+    int grid;       // workgroups of 256 threads (persistent)
+    size_t lds_bytes;
+};
+
+hipError_t launch_solve(const LaunchSpec &spec, const KArgs &args, hipStream_t stream);
+hipError_t launch_walk(int gap, const Prob *probs, int64_t n, const uint32_t *tb, const int32_t *seeds,
+                       int32_t *path, const int64_t *path_off, int32_t *path_len, hipStream_t stream);
+hipError_t launch_zero_u32(uint32_t *p, int64_t n, hipStream_t stream);
+bool supported_R(int form, int tr, bool fastmax, int R);
+int occupancy_waves(const LaunchSpec &spec);  // resident waves per CU for this instantiation
+
+// Traceback bits per cell and 32-bit words per lane per step.
+inline int tb_bits(int gap, int tr) { return tr == TR_ALL ? 8 : (tr == TR_ONE ? (gap == GAP_AFF ? 4 : 2) : 0); }
+inline int tb_words(int gap, int tr, int R) { return (tb_bits(gap, tr) * R + 31) / 32; }
+
+}  // namespace pa

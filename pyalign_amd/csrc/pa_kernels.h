@@ -1,0 +1,510 @@
+// pa_kernels.h -- the gfx950 DP kernel template (included by the pa_k_*.hip
+// translation units, which instantiate disjoint subsets in parallel builds).
+//
+// Work decomposition.  A wavefront takes one group from a cost-sorted work
+// queue.  A group holds 64/L problems; L consecutive lanes (a segment)
+// cooperate on one problem.  Lane l of a segment owns R consecutive rows of
+// the current strip (L*R rows); at step s it processes column j = s - l of
+// its R rows, so the segment sweeps the matrix along anti-diagonals.  The
+// bottom row's D and P of lane l reach lane l+1 one step later through a
+// DPP wave shift (wave_shr:1); lane 0 of a segment reads the row above the
+// strip from the border or, for long problems, from the row buffer written
+// by the segment's last lane in the previous strip.  L = 1 is the
+// inter-task mode (one problem per lane), L = 64 the intra-task mode (one
+// problem per wavefront); the host autotuner picks (L, R) per problem.
+//
+// Every cell performs the reference's candidate fold in the reference's
+// order with strict-improvement updates (pyalign/algorithm/solver.h:1311-1427;
+// linear 2837-2854, Gotoh 2989-3035, DTW 3233-3250), so values are
+// bit-identical to the reference's float32 arithmetic.  The `FAST` variant
+// (local maximisation, score only, NaN-free table) uses v_max_f32: there
+// every value is >= +0 and no -0 can arise (the zero candidate is first and
+// a -0 never beats it, and +0 + -0 = +0), so max returns exactly the value
+// the strict select keeps.
+#pragma once
+
+#include "pa_internal.h"
+
+namespace pa {
+
+template <int DIR>
+__device__ __forceinline__ bool better(float a, float b) {
+    if constexpr (DIR == DIR_MAX) return a > b;
+    else return a < b;
+}
+
+template <int DIR>
+__device__ __forceinline__ constexpr float worst() {
+    return DIR == DIR_MAX ? -__builtin_inff() : __builtin_inff();
+}
+
+// lane k <- lane k-1 (wave_shr:1); lane 0 keeps its own value
+__device__ __forceinline__ float shr1(float x) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+
+// Problems with an empty side: the seed comes from the border
+// (solver.h:2042-2057, 2179-2187, 2273-2293).
+template <int LOC, int DIR>
+__device__ void write_empty(const KArgs &a, const Prob &pr) {
+    const int ls = pr.len_s, lt = pr.len_t;
+    float score = worst<DIR>();
+    int si = NO_SEED, sj = 0;
+    if (LOC == LOC_GLOBAL) {
+        si = ls - 1;
+        sj = lt - 1;
+        score = ls == 0 ? a.border_t[lt] : a.border_s[ls];
+    } else if (LOC == LOC_SEMI) {
+        if (ls == 0 && lt > 0) { si = -1; sj = 0; score = a.border_t[1]; }
+        else if (lt == 0 && ls > 0) { si = 0; sj = -1; score = a.border_s[1]; }
+    }
+    a.scores[pr.out] = score;
+    if (a.seeds) {
+        a.seeds[2 * pr.out] = si;
+        a.seeds[2 * pr.out + 1] = sj;
+    }
+}
+
+template <int LOC, int DIR, int GAP, int FORM, int TR, int R, bool FAST, int SD>
+__device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, const float *__restrict__ tab,
+                                            int lane) {
+    constexpr float W = worst<DIR>();
+    constexpr int BITS = TR == TR_ALL ? 8 : (TR == TR_ONE ? (GAP == GAP_AFF ? 4 : 2) : 0);
+    constexpr int WPC = TR == TR_SCORE ? 1 : (BITS * R + 31) / 32;
+    constexpr int SDD = SD > 0 ? SD : 1;
+
+    const int L = g.L;
+    const int seg = lane >> g.lgL;
+    const int l = lane & (L - 1);
+    const bool has = seg < g.count;
+    Prob pr;
+    if (has) {
+        pr = a.probs[g.first + seg];
+    } else {
+        pr.off_s = pr.off_t = pr.tb_off = pr.rb_off = pr.dv_off = 0;
+        pr.len_s = pr.len_t = 0;
+        pr.si = pr.sj = pr.out = 0;
+    }
+    const int ls = pr.len_s, lt = pr.len_t;
+    const int rps = L * R;
+    const int nst = (ls > 0 && lt > 0) ? (ls + rps - 1) / rps : 0;
+    const int steps = g.max_lt + L - 1;
+    const int nsteps_p = lt + L - 1;  // this problem's steps per strip (traceback layout)
+    const int PAD = a.K1 - 1;
+    const float go_s = a.go_s, ge_s = a.ge_s, go_t = a.go_t, ge_t = a.ge_t;
+
+    // ---- seed / score tracking (per lane, reduced over the segment) -------
+    float best = W;
+    int bi = -1, bj = -1;
+    float rbest = W, cbest = W;
+    int rj = -1, ci = -1;
+    float gscore = W;
+    const int last = ls - 1;
+    const int st_last = last >= 0 ? last / rps : -1;
+    const int l_last = last >= 0 ? (last % rps) / R : -1;
+    const int r_last = last >= 0 ? last % R : -1;
+
+    for (int st = 0; st < g.max_nst; ++st) {
+        const bool act = st < nst;
+        const int row0 = st * rps + l * R;
+
+        int rowsym[R];
+        bool rv[R];
+        int64_t rowbase[R];
+        float sa[R][SDD];
+        float Dl[R], Ql[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = row0 + r;
+            rv[r] = act && i < ls;
+            if constexpr (FORM == F_TABLE) {
+                const uint8_t *cs = (const uint8_t *)a.codes_s;
+                rowsym[r] = (rv[r] ? int(cs[pr.off_s + i]) : PAD) * a.K1;
+            } else if constexpr (FORM == F_BINARY) {
+                const uint32_t *cs = (const uint32_t *)a.codes_s;
+                rowsym[r] = rv[r] ? int(cs[pr.off_s + i]) : 0;
+            } else if constexpr (FORM == F_DENSE) {
+                rowbase[r] = pr.off_s + int64_t(rv[r] ? i : 0) * pr.si;
+            } else if constexpr (FORM == F_SERIES) {
+                const float *xs = (const float *)a.codes_s;
+#pragma unroll
+                for (int d = 0; d < SDD; ++d) sa[r][d] = rv[r] ? xs[(pr.off_s + i) * SD + d] : 0.0f;
+            }
+            Dl[r] = rv[r] ? a.border_s[i + 1] : W;  // D(i, -1)
+            Ql[r] = W;                              // Q(i, -1) (solver.h:2926-2933)
+        }
+        float botD = Dl[R - 1], botP = W;  // this lane's last row, handed down
+        // diagonal predecessor of the strip's first row at j = 0: D(row0-1, -1)
+        float prevD = W;
+        if (l == 0) prevD = (row0 == 0) ? a.border_t[0] : (act ? a.border_s[row0] : W);
+
+        for (int s = 0; s < steps; ++s) {
+            const int j = s - l;
+            float inD = shr1(botD), inP = shr1(botP);
+            if (l == 0) {
+                if (st == 0) {
+                    inD = (act && j < lt) ? a.border_t[j + 1] : W;  // D(-1, j)
+                    inP = W;                                        // P(-1, j)
+                } else if (act && j < lt) {
+                    const float2 v = a.rowbuf[pr.rb_off + j];
+                    inD = v.x;
+                    inP = v.y;
+                }
+            }
+            const float Ddiag = prevD;
+            prevD = inD;
+            if (act && j >= 0 && j < lt) {
+                int tsym = 0;
+                int64_t coloff = 0;
+                float tv[SDD];
+                if constexpr (FORM == F_TABLE) {
+                    tsym = int(((const uint8_t *)a.codes_t)[pr.off_t + j]);
+                } else if constexpr (FORM == F_BINARY) {
+                    tsym = int(((const uint32_t *)a.codes_t)[pr.off_t + j]);
+                } else if constexpr (FORM == F_DENSE) {
+                    coloff = int64_t(j) * pr.sj;
+                } else if constexpr (FORM == F_SERIES) {
+                    const float *xt = (const float *)a.codes_t;
+#pragma unroll
+                    for (int d = 0; d < SDD; ++d) tv[d] = xt[(pr.off_t + j) * SD + d];
+                }
+                float Dd = Ddiag, Dup = inD, Pup = inP;
+                uint32_t tbw[WPC];
+#pragma unroll
+                for (int w = 0; w < WPC; ++w) tbw[w] = 0;
+
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    // similarity S(i, j) (factory.h:573-576, 624-630, 673-679)
+                    float S;
+                    if constexpr (FORM == F_TABLE) {
+                        S = tab[rowsym[r] + tsym];
+                    } else if constexpr (FORM == F_BINARY) {
+                        S = rv[r] ? (rowsym[r] == tsym ? a.eq : a.ne) : W;
+                    } else if constexpr (FORM == F_DENSE) {
+                        S = rv[r] ? a.dense[rowbase[r] + coloff] : W;
+                    } else {
+                        // squared Euclidean cost, dimensions summed in order
+                        float c = 0.0f;
+#pragma unroll
+                        for (int d = 0; d < SDD; ++d) {
+                            const float df = sa[r][d] - tv[d];
+                            c = (d == 0) ? df * df : c + df * df;
+                        }
+                        S = rv[r] ? c : W;
+                    }
+
+                    float D, P = 0.0f, Q = 0.0f;
+                    uint32_t bits = 0;
+                    if constexpr (GAP == GAP_DTW) {
+                        // D = opt(diag, up, left) + cost (solver.h:3236-3248)
+                        float m = Dd;
+                        int src = 1;
+                        if (better<DIR>(Dup, m)) { m = Dup; src = 2; }
+                        if (better<DIR>(Dl[r], m)) { m = Dl[r]; src = 3; }
+                        D = m + S;
+                        bits = uint32_t(src);
+                    } else if constexpr (GAP == GAP_LIN) {
+                        // [0], diag, up, left (solver.h:2841-2852)
+                        const float dg = Dd + S;
+                        const float up = Dup + go_s;
+                        const float lf = Dl[r] + go_t;
+                        if constexpr (FAST) {
+                            D = fmaxf(fmaxf(fmaxf(dg, 0.0f), up), lf);
+                        } else {
+                            int src;
+                            if constexpr (LOC == LOC_LOCAL) {
+                                D = 0.0f;
+                                src = 0;
+                                if (better<DIR>(dg, D)) { D = dg; src = 1; }
+                            } else {
+                                D = dg;
+                                src = 1;
+                            }
+                            if (better<DIR>(up, D)) { D = up; src = 2; }
+                            if (better<DIR>(lf, D)) { D = lf; src = 3; }
+                            if constexpr (TR == TR_ONE) bits = uint32_t(src);
+                            if constexpr (TR == TR_ALL)
+                                bits = (LOC == LOC_LOCAL && D == 0.0f ? 1u : 0u) | (dg == D ? 2u : 0u) |
+                                       (up == D ? 4u : 0u) | (lf == D ? 8u : 0u);
+                        }
+                    } else {
+                        // Gotoh (solver.h:2993-3033)
+                        const float po = Dup + go_s, pe = Pup + ge_s;
+                        const float qo = Dl[r] + go_t, qe = Ql[r] + ge_t;
+                        const float dg = Dd + S;
+                        if constexpr (FAST) {
+                            P = fmaxf(po, pe);
+                            Q = fmaxf(qo, qe);
+                            D = fmaxf(fmaxf(fmaxf(dg, 0.0f), P), Q);
+                        } else {
+                            const bool pext = better<DIR>(pe, po);
+                            P = pext ? pe : po;
+                            const bool qext = better<DIR>(qe, qo);
+                            Q = qext ? qe : qo;
+                            int src;
+                            if constexpr (LOC == LOC_LOCAL) {
+                                D = 0.0f;
+                                src = 0;
+                                if (better<DIR>(dg, D)) { D = dg; src = 1; }
+                            } else {
+                                D = dg;
+                                src = 1;
+                            }
+                            if (better<DIR>(P, D)) { D = P; src = 2; }
+                            if (better<DIR>(Q, D)) { D = Q; src = 3; }
+                            if constexpr (TR == TR_ONE)
+                                bits = uint32_t(src) | (pext ? 4u : 0u) | (qext ? 8u : 0u);
+                            if constexpr (TR == TR_ALL)
+                                bits = (LOC == LOC_LOCAL && D == 0.0f ? 1u : 0u) | (dg == D ? 2u : 0u) |
+                                       (P == D ? 4u : 0u) | (Q == D ? 8u : 0u) | (po == P ? 16u : 0u) |
+                                       (pe == P ? 32u : 0u) | (qo == Q ? 64u : 0u) | (qe == Q ? 128u : 0u);
+                        }
+                    }
+
+                    // ---- seeds ------------------------------------------
+                    const int i = row0 + r;
+                    if constexpr (LOC == LOC_LOCAL) {
+                        if constexpr (TR == TR_SCORE) {
+                            // pad rows never exceed the optimum: their S is the
+                            // worst value and gap costs are non-negative
+                            if constexpr (FAST) best = fmaxf(best, D);
+                            else best = better<DIR>(D, best) ? D : best;
+                        } else {
+                            // Optima over i desc, j desc, first strict optimum
+                            // wins (solver.h:2050-2056): largest i, then largest j
+                            if (rv[r] && (better<DIR>(D, best) || (D == best && i >= bi))) {
+                                best = D;
+                                bi = i;
+                                bj = j;
+                            }
+                            if constexpr (TR == TR_ALL) {
+                                if (rv[r]) a.dvals[pr.dv_off + int64_t(i) * lt + j] = D;
+                            }
+                        }
+                    }
+                    if constexpr (TR != TR_SCORE) tbw[(r * BITS) / 32] |= bits << ((r * BITS) % 32);
+
+                    Dd = Dl[r];
+                    Dl[r] = D;
+                    Ql[r] = Q;
+                    Dup = D;
+                    Pup = P;
+                }
+                botD = Dup;
+                botP = Pup;
+                if (l == L - 1 && st + 1 < nst) a.rowbuf[pr.rb_off + j] = make_float2(botD, botP);
+
+                if constexpr (TR != TR_SCORE) {
+                    uint32_t *dst = a.tb + pr.tb_off + (int64_t(st) * nsteps_p + s) * int64_t(L) * WPC + l * WPC;
+#pragma unroll
+                    for (int w = 0; w < WPC; ++w) dst[w] = tbw[w];
+                }
+
+                // ---- seeds from the finished column (global / semiglobal) --
+                if constexpr (LOC == LOC_SEMI || LOC == LOC_GLOBAL) {
+                    if (st == st_last && l == l_last) {
+                        float v = Dl[0];
+#pragma unroll
+                        for (int r = 1; r < R; ++r) v = (r == r_last) ? Dl[r] : v;
+                        if constexpr (LOC == LOC_SEMI) {
+                            // last row, j ascending, strict (solver.h:2285-2287)
+                            if (better<DIR>(v, rbest)) { rbest = v; rj = j; }
+                        } else {
+                            if (j == lt - 1) gscore = v;
+                        }
+                    }
+                    if constexpr (LOC == LOC_SEMI) {
+                        if (j == lt - 1) {
+                            // last column, i ascending, strict (solver.h:2289-2291)
+#pragma unroll
+                            for (int r = 0; r < R; ++r)
+                                if (rv[r] && better<DIR>(Dl[r], cbest)) { cbest = Dl[r]; ci = row0 + r; }
+                        }
+                    }
+                }
+            }
+        }
+        // the row buffer written by this strip's last lane is read next strip
+        __threadfence_block();
+    }
+
+    // ---- reduce over the segment's lanes, lane 0 writes -----------------
+    float score = W;
+    int si = NO_SEED, sj = 0;
+    if constexpr (LOC == LOC_LOCAL) {
+        for (int m = 1; m < L; m <<= 1) {
+            const float ob = __shfl_xor(best, m, WAVE);
+            if constexpr (TR == TR_SCORE) {
+                if constexpr (FAST) best = fmaxf(best, ob);
+                else best = better<DIR>(ob, best) ? ob : best;
+            } else {
+                const int oi = __shfl_xor(bi, m, WAVE);
+                const int oj = __shfl_xor(bj, m, WAVE);
+                if (better<DIR>(ob, best) || (ob == best && oi > bi)) {
+                    best = ob;
+                    bi = oi;
+                    bj = oj;
+                }
+            }
+        }
+        score = best;
+        si = bi;
+        sj = bj;
+    } else if constexpr (LOC == LOC_GLOBAL) {
+        const int src = (seg << g.lgL) + (l_last > 0 ? l_last : 0);
+        score = __shfl(gscore, src, WAVE);
+        si = ls - 1;
+        sj = lt - 1;
+    } else {
+        const int src = (seg << g.lgL) + (l_last > 0 ? l_last : 0);
+        rbest = __shfl(rbest, src, WAVE);
+        rj = __shfl(rj, src, WAVE);
+        for (int m = 1; m < L; m <<= 1) {
+            const float oc = __shfl_xor(cbest, m, WAVE);
+            const int oi = __shfl_xor(ci, m, WAVE);
+            if (better<DIR>(oc, cbest) || (oc == cbest && oi >= 0 && (ci < 0 || oi < ci))) {
+                cbest = oc;
+                ci = oi;
+            }
+        }
+        // Optima: the row scan first; the column scan replaces only on strict
+        // improvement
+        if (better<DIR>(cbest, rbest)) { score = cbest; si = ci; sj = lt - 1; }
+        else { score = rbest; si = ls - 1; sj = rj; }
+    }
+    if (has && l == 0) {
+        if (ls == 0 || lt == 0) {
+            write_empty<LOC, DIR>(a, pr);
+        } else {
+            if (LOC == LOC_LOCAL && TR == TR_SCORE) { si = 0; sj = 0; }
+            a.scores[pr.out] = score;
+            if (a.seeds) {
+                a.seeds[2 * pr.out] = si;
+                a.seeds[2 * pr.out + 1] = sj;
+            }
+        }
+    }
+}
+
+template <int LOC, int DIR, int GAP, int FORM, int TR, int R, bool FAST, int SD>
+__global__ __launch_bounds__(256) void pa_solve_kernel(KArgs a) {
+    extern __shared__ float lds_table[];
+    if constexpr (FORM == F_TABLE) {
+        const int n = a.K1 * a.K1;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) lds_table[k] = a.table[k];
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & (WAVE - 1);
+    // Persistent wavefronts drain the cost-sorted group queue; every wave
+    // leaves once its ticket passes the group count.
+    while (true) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(a.queue, 1u);
+        t = uint32_t(__shfl(int(t), 0, WAVE));
+        if (t >= uint32_t(a.n_groups)) break;
+        solve_group<LOC, DIR, GAP, FORM, TR, R, FAST, SD>(a, a.groups[t], lds_table, lane);
+    }
+}
+
+template <int LOC, int DIR, int GAP, int FORM, int TR, int R, bool FAST, int SD>
+static hipError_t go(const LaunchSpec &s, const KArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((pa_solve_kernel<LOC, DIR, GAP, FORM, TR, R, FAST, SD>), dim3(s.grid), dim3(256),
+                       s.lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+template <int LOC, int DIR, int GAP, int FORM, int TR, int R, bool FAST, int SD>
+static int occ(const LaunchSpec &s) {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks, reinterpret_cast<const void *>(&pa_solve_kernel<LOC, DIR, GAP, FORM, TR, R, FAST, SD>), 256,
+            s.lds_bytes) != hipSuccess)
+        return 4;
+    return blocks * 4;
+}
+
+// Generic dispatch for one (LOC, DIR, GAP): forms x traceback modes at the
+// general strip height R = 8; the hot score-only table path gets more.
+template <int LOC, int DIR, int GAP, bool OCC>
+static hipError_t dispatch_lg(const LaunchSpec &s, const KArgs &a, hipStream_t st, int *occ_out) {
+#define PA_GO(FORM, TR, R, FAST, SD)                                                        \
+    do {                                                                                    \
+        if (OCC) {                                                                          \
+            *occ_out = occ<LOC, DIR, GAP, FORM, TR, R, FAST, SD>(s);                        \
+            return hipSuccess;                                                              \
+        }                                                                                   \
+        return go<LOC, DIR, GAP, FORM, TR, R, FAST, SD>(s, a, st);                          \
+    } while (0)
+    if constexpr (GAP == GAP_DTW) {
+        if (s.tr == TR_ALL) return hipErrorInvalidValue;
+        if (s.R != 8) return hipErrorInvalidValue;
+        if (s.form == F_DENSE) {
+            if (s.tr == TR_SCORE) PA_GO(F_DENSE, TR_SCORE, 8, false, 0);
+            PA_GO(F_DENSE, TR_ONE, 8, false, 0);
+        }
+        if (s.form == F_SERIES) {
+            switch (a.dim) {
+                case 1: if (s.tr == TR_SCORE) PA_GO(F_SERIES, TR_SCORE, 8, false, 1); PA_GO(F_SERIES, TR_ONE, 8, false, 1);
+                case 2: if (s.tr == TR_SCORE) PA_GO(F_SERIES, TR_SCORE, 8, false, 2); PA_GO(F_SERIES, TR_ONE, 8, false, 2);
+                case 3: if (s.tr == TR_SCORE) PA_GO(F_SERIES, TR_SCORE, 8, false, 3); PA_GO(F_SERIES, TR_ONE, 8, false, 3);
+                case 4: if (s.tr == TR_SCORE) PA_GO(F_SERIES, TR_SCORE, 8, false, 4); PA_GO(F_SERIES, TR_ONE, 8, false, 4);
+                default: return hipErrorInvalidValue;
+            }
+        }
+        return hipErrorInvalidValue;
+    } else {
+        if (s.form == F_TABLE && s.tr == TR_SCORE) {
+            if (s.fastmax) {
+                if constexpr (LOC == LOC_LOCAL && DIR == DIR_MAX) {
+                    switch (s.R) {
+                        case 1: PA_GO(F_TABLE, TR_SCORE, 1, true, 0);
+                        case 2: PA_GO(F_TABLE, TR_SCORE, 2, true, 0);
+                        case 3: PA_GO(F_TABLE, TR_SCORE, 3, true, 0);
+                        case 4: PA_GO(F_TABLE, TR_SCORE, 4, true, 0);
+                        case 5: PA_GO(F_TABLE, TR_SCORE, 5, true, 0);
+                        case 6: PA_GO(F_TABLE, TR_SCORE, 6, true, 0);
+                        case 7: PA_GO(F_TABLE, TR_SCORE, 7, true, 0);
+                        case 8: PA_GO(F_TABLE, TR_SCORE, 8, true, 0);
+                        case 10: PA_GO(F_TABLE, TR_SCORE, 10, true, 0);
+                        case 12: PA_GO(F_TABLE, TR_SCORE, 12, true, 0);
+                        case 14: PA_GO(F_TABLE, TR_SCORE, 14, true, 0);
+                        case 16: PA_GO(F_TABLE, TR_SCORE, 16, true, 0);
+                        case 20: PA_GO(F_TABLE, TR_SCORE, 20, true, 0);
+                        case 24: PA_GO(F_TABLE, TR_SCORE, 24, true, 0);
+                        case 28: PA_GO(F_TABLE, TR_SCORE, 28, true, 0);
+                        case 32: PA_GO(F_TABLE, TR_SCORE, 32, true, 0);
+                        default: return hipErrorInvalidValue;
+                    }
+                }
+                return hipErrorInvalidValue;
+            }
+            switch (s.R) {
+                case 4: PA_GO(F_TABLE, TR_SCORE, 4, false, 0);
+                case 8: PA_GO(F_TABLE, TR_SCORE, 8, false, 0);
+                case 16: PA_GO(F_TABLE, TR_SCORE, 16, false, 0);
+                case 32: PA_GO(F_TABLE, TR_SCORE, 32, false, 0);
+                default: return hipErrorInvalidValue;
+            }
+        }
+        if (s.R != 8) return hipErrorInvalidValue;
+        switch (s.form) {
+            case F_TABLE:
+                if (s.tr == TR_ONE) PA_GO(F_TABLE, TR_ONE, 8, false, 0);
+                PA_GO(F_TABLE, TR_ALL, 8, false, 0);
+            case F_BINARY:
+                if (s.tr == TR_SCORE) PA_GO(F_BINARY, TR_SCORE, 8, false, 0);
+                if (s.tr == TR_ONE) PA_GO(F_BINARY, TR_ONE, 8, false, 0);
+                PA_GO(F_BINARY, TR_ALL, 8, false, 0);
+            case F_DENSE:
+                if (s.tr == TR_SCORE) PA_GO(F_DENSE, TR_SCORE, 8, false, 0);
+                if (s.tr == TR_ONE) PA_GO(F_DENSE, TR_ONE, 8, false, 0);
+                PA_GO(F_DENSE, TR_ALL, 8, false, 0);
+            default: return hipErrorInvalidValue;
+        }
+    }
+#undef PA_GO
+}
+
+}  // namespace pa
