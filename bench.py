@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Batched local Gotoh alignment throughput (GCUPS) on one MI355X.
+
+Workload: 65,536 pairs, each length floor(exp(U(ln 50, ln 2000))) drawn
+independently, 20-letter alphabet, symmetric integer substitution table
+(diagonal U[4,11], off-diagonal U[-4,3]) as float32, AffineGapCost(open=10,
+extend=1), local alignment, score only.  Seed 2.  Sequences, table and
+output scores are resident in HBM before timing starts.
+
+A step solves the whole batch once (one persistent, autotuned launch per
+strip height).  Reported beside it: the plain GPU form (the reference's own
+Morton batches, one problem per lane, no work queue or autotuning) on the
+same inputs with a bitwise check of the scores, and the CPU baseline (the
+oracle's 16-lane restatement of the reference's batched path, timed on the
+host's cores on a bounded sample of the same workload).
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GCUPS batched Gotoh local align, fwd & fwd+bwd, vs plain GPU path; CPU ref"
+OPS_PER_CELL = 11          # 5 add + 4 max + local zero clamp + running best (Gotoh, local score)
+VALU_PEAK_TOPS = 157.3 / 4  # MI355X fp32 vector peak counts packed FMA (x2 x2); one lane-op/clk here
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def c2_inputs(n=65536, seed=2):
+    rng = np.random.default_rng(seed)
+    K = 20
+    table = rng.integers(-4, 4, size=(K, K))
+    table = np.triu(table) + np.triu(table, 1).T
+    np.fill_diagonal(table, rng.integers(4, 12, size=K))
+    table = table.astype(np.float32)
+    ls = np.floor(np.exp(rng.uniform(np.log(50), np.log(2000), n))).astype(np.int32)
+    lt = np.floor(np.exp(rng.uniform(np.log(50), np.log(2000), n))).astype(np.int32)
+    codes_s = rng.integers(0, K, int(ls.sum()), dtype=np.uint8)
+    codes_t = rng.integers(0, K, int(lt.sum()), dtype=np.uint8)
+    off_s = np.zeros(n, np.int64)
+    off_t = np.zeros(n, np.int64)
+    off_s[1:] = np.cumsum(ls.astype(np.int64))[:-1]
+    off_t[1:] = np.cumsum(lt.astype(np.int64))[:-1]
+    return table, ls, lt, codes_s, codes_t, off_s, off_t
+
+
+def traffic():
+    """HBM bytes per launch of the score kernel from the committed PMC runs
+    (FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "pa_fast_kernel_c2_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return round(json.load(f)["traffic_bytes"])
+
+
+def make_solver(lim_s, lim_t):
+    import pyalign_amd as pa
+    from pyalign_amd import algorithm
+    opts = algorithm.m16.create_options(dict(
+        solver=algorithm.Type.ALIGNMENT, batch=True, direction=algorithm.Direction.MAXIMIZE, return_dup=False,
+        codomain=pa.solve.Codomain(pa.Score), locality=algorithm.Locality.LOCAL,
+        gap_cost=pa.AffineGapCost(10, 1)))
+    return algorithm.m16.create_solver(lim_s, lim_t, opts)
+
+
+def cpu_baseline(table, ls, lt, codes_s, codes_t, off_s, off_t, sample, lim):
+    """The oracle's 16-lane batched score path (reference batching) on the host."""
+    import oracle
+    import pyalign_amd as pa
+
+    class _Shape:
+        form = pa.Form.INDEXED_MATRIX_FORM
+        direction = "maximize"
+        dtype = np.float32
+
+        def __init__(self, a, b):
+            self.shape = (a, b)
+
+    idx = np.arange(sample)
+    bag = pa.ProblemBag([_Shape(int(ls[k]), int(lt[k])) for k in idx])
+    members, boff = [], [0]
+    for b in bag.batches(16):  # instance.py:230-275 with 16 AVX-512 lanes
+        members += list(b.indices)
+        boff.append(len(members))
+    threads = min(16, os.cpu_count() or 1)
+    cfg = oracle.make_config(locality=oracle.LOCAL, gap=("affine", 10, 1), goal=oracle.GOAL_SCORE,
+                             lim_s=lim, lim_t=lim)
+    secs, scores = oracle.baseline_indexed_score(
+        cfg, codes_s, off_s[:sample], ls[:sample].astype(np.int64), codes_t, off_t[:sample],
+        lt[:sample].astype(np.int64), table, np.array(members), np.array(boff), threads)
+    cells = float((ls[:sample].astype(np.float64) * lt[:sample]).sum())
+    return cells / secs / 1e9, threads, secs, scores
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run a float32 elementwise copy of this many bytes (PMC byte-counter scale)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
+              "(independent problems would shard as contiguous cost-balanced ranges, one per GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pyalign_amd import _lib, algorithm
+    from pyalign_amd._lib import check, lib
+
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.add(src, 0.0, out=dst)  # reads and writes calibrate_copy bytes once
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
+        return
+    table, ls, lt, codes_s, codes_t, off_s, off_t = c2_inputs()
+    n = len(ls)
+    cells = float((ls.astype(np.float64) * lt).sum())
+    lim = int(max(ls.max(), lt.max()))  # SizeRounder keeps limits >= 1024 exact
+    solver = make_solver(lim, lim)
+
+    # inputs resident in HBM
+    d_cs = torch.from_numpy(codes_s).cuda()
+    d_ct = torch.from_numpy(codes_t).cuda()
+    d_scores = torch.empty(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    rg, keep = algorithm.make_ragged("indexed", ls, lt, codes_s=d_cs.data_ptr(), codes_t=d_ct.data_ptr(),
+                                     off_s=off_s, off_t=off_t, table=table, on_device=True)
+
+    def step(schedule):
+        check(lib().pa_solve_ragged_scores_device(solver._h, ctypes.byref(rg), schedule,
+                                                  ctypes.c_void_p(d_scores.data_ptr())))
+        check(lib().pa_solver_sync(solver._h))
+        ms, launches, _ = solver.last_kernel_ms()
+        return ms, launches
+
+    log(f"inputs ready: {n} pairs, {cells:.3e} cells")
+    for _ in range(args.warmup):
+        step(_lib.PA_SCHED_AUTO)
+    log("warmup done")
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ms, launches = step(_lib.PA_SCHED_AUTO)
+        kernel_ms.append(ms)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    auto_scores = d_scores.cpu().numpy().copy()
+    value = cells / (ms_per_step * 1e-3) / 1e9
+    kms = float(np.mean(kernel_ms))
+    achieved = cells * OPS_PER_CELL / (kms * 1e-3) / 1e12
+
+    log(f"batched: {ms_per_step:.2f} ms/step, {value:.1f} GCUPS")
+    plain = None
+    if not args.no_plain:
+        step(_lib.PA_SCHED_PLAIN)
+        pms = []
+        tp0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            pms.append(step(_lib.PA_SCHED_PLAIN)[0])
+        torch.cuda.synchronize()
+        plain_step = (time.perf_counter() - tp0) * 1e3 / args.plain_steps
+        plain_scores = d_scores.cpu().numpy()
+        plain = {"value": round(cells / (plain_step * 1e-3) / 1e9, 3), "unit": "GCUPS",
+                 "ms_per_step": round(plain_step, 3), "kernel_ms": round(float(np.mean(pms)), 3),
+                 "scores_bitwise_equal": bool(np.array_equal(plain_scores.view(np.uint32),
+                                                             auto_scores.view(np.uint32))),
+                 "speedup_of_batched": round(value / (cells / (plain_step * 1e-3) / 1e9), 2)}
+
+    if plain:
+        log(f"plain: {plain['ms_per_step']:.2f} ms/step")
+    cpu = None
+    if not args.no_cpu:
+        sample = min(args.cpu_sample, n)
+        gcups, threads, secs, cpu_scores = cpu_baseline(table, ls, lt, codes_s, codes_t, off_s, off_t, sample, lim)
+        cpu = {"value": round(gcups, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+               "sample": (f"all {n} pairs" if sample == n else f"first {sample} of the {n} pairs")
+                         + f" ({secs:.1f} s; oracle's 16-lane batched restatement, Morton batches of 16 as the "
+                           "reference's ProblemBag.batches)",
+               "scores_bitwise_equal": bool(np.array_equal(cpu_scores.view(np.uint32),
+                                                           auto_scores[:sample].view(np.uint32)))}
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched local Gotoh score-only: 65536 pairs, lengths log-uniform [50,2000], "
+                               "20-letter integer table, affine open 10 extend 1",
+                   "pairs": n, "cells": cells, "forward": "score", "launches_per_step": launches},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 3),
+                     "unit": "Tops/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": traffic(),
+                     "ops_per_cell": OPS_PER_CELL, "kernel_ms_per_step": round(kms, 3)},
+        "plain_gpu": plain,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -286,12 +286,66 @@ public:
     // ---- DP solve over the batch's master shape -----------------------
     // pairwise(i, j, out[B]) gives S(i,j) per lane (factory.h:573-576,
     // 624-630, 673-679).
+    // Value-only accumulation (Accumulator, solver.h:1226-1309): lane-wise
+    // cur = opt(x, cur) keeping the old value on ties.
+    // Lane vectors as in the reference's tiny_vector/xsimd batches
+    // (solver.h:376-422): one SIMD register of B float32 lanes.
+    typedef float VecB __attribute__((vector_size(sizeof(float) * B)));
+    static inline VecB ld(const float *p) { VecB v; std::memcpy(&v, p, sizeof(VecB)); return v; }
+    static inline void st(float *p, VecB v) { std::memcpy(p, &v, sizeof(VecB)); }
+    static inline VecB vopt(int dir, VecB x, VecB cur) {
+        // strict improvement, old value kept on ties (xt::maximum / minimum)
+        return dir == D_MAXIMIZE ? (x > cur ? x : cur) : (x < cur ? x : cur);
+    }
+
+    template <typename Pairwise>
+    void solve_values(const Pairwise &pairwise, int64_t mls, int64_t mlt) {
+        alignas(64) float S[B];
+        const int dir = cfg.direction;
+        const bool local = cfg.locality == L_LOCAL;
+        const VecB zero = VecB{} * 0.0f;
+        if (cfg.gap_kind == GAP_LINEAR) {
+            const VecB gs = zero + cfg.lin_s * sgn, gt = zero + cfg.lin_t * sgn;
+            for (int64_t u = 0; u < mls; u++) {
+                for (int64_t v = 0; v < mlt; v++) {
+                    pairwise(u, v, S);
+                    const VecB x = ld(&val[idx(0, u - 1, v - 1) * B]) + ld(S);
+                    VecB m = local ? vopt(dir, x, zero) : x;
+                    m = vopt(dir, ld(&val[idx(0, u - 1, v) * B]) + gs, m);
+                    m = vopt(dir, ld(&val[idx(0, u, v - 1) * B]) + gt, m);
+                    st(&val[idx(0, u, v) * B], m);
+                }
+            }
+            return;
+        }
+        const VecB os = zero + w1_s * sgn, es = zero + ext_s * sgn;
+        const VecB ot = zero + w1_t * sgn, et = zero + ext_t * sgn;
+        for (int64_t i = 0; i < mls; i++) {
+            for (int64_t j = 0; j < mlt; j++) {
+                pairwise(i, j, S);
+                const VecB P = vopt(dir, ld(&val[idx(1, i - 1, j) * B]) + es, ld(&val[idx(0, i - 1, j) * B]) + os);
+                const VecB Q = vopt(dir, ld(&val[idx(2, i, j - 1) * B]) + et, ld(&val[idx(0, i, j - 1) * B]) + ot);
+                const VecB x = ld(&val[idx(0, i - 1, j - 1) * B]) + ld(S);
+                VecB m = local ? vopt(dir, x, zero) : x;
+                m = vopt(dir, P, m);
+                m = vopt(dir, Q, m);
+                st(&val[idx(1, i, j) * B], P);
+                st(&val[idx(2, i, j) * B], Q);
+                st(&val[idx(0, i, j) * B], m);
+            }
+        }
+    }
+
     template <typename Pairwise>
     void solve(const Pairwise &pairwise, int64_t mls, int64_t mlt,
                const int64_t *len_s, const int64_t *len_t) {
         if (mls > cfg.lim_s || mlt > cfg.lim_t)
             throw std::runtime_error("exceeded_configured_length");  // solver.h:134-152
         for (int b = 0; b < B; b++) { cur_len_s[b] = len_s[b]; cur_len_t[b] = len_t[b]; }
+        if (!trace && cfg.type == T_ALIGNMENT) {
+            solve_values(pairwise, mls, mlt);
+            return;
+        }
         float S[B], x[B];
         const int dir = cfg.direction;
         if (cfg.type == T_DTW) {

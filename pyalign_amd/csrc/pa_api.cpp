@@ -129,8 +129,35 @@ struct pa_options {
     int index_bits = 16;
 };
 
+namespace {
+
+struct Plan {
+    std::vector<Prob> probs;    // in group order
+    std::vector<Group> groups;  // per launch, concatenated
+    std::vector<std::pair<int, std::pair<int, int>>> launches;  // R, [first group, end group)
+    int64_t rowbuf = 0;         // float2 elements
+    int64_t tb_words = 0;
+    int64_t dvals = 0;
+    double cells = 0.0;
+};
+
+// The last call's schedule, reused (with its device copy) while the next
+// call has the same lengths, offsets and job: repeated batch shapes skip
+// planning and the descriptor upload.
+struct PlanCache {
+    bool valid = false;
+    std::vector<int64_t> key;
+    std::vector<int32_t> len_s, len_t;
+    std::vector<int64_t> off_s, off_t;
+    Plan plan;
+};
+
+}  // namespace
+
 struct pa_solver {
     pa_options opt;
+    PlanCache last_plan;
+    std::vector<std::pair<std::vector<int>, int>> occ_cache;
     int64_t lim_s = 0, lim_t = 0;
     std::vector<float> border_s, border_t;
     float go_s = 0, ge_s = 0, go_t = 0, ge_t = 0;
@@ -244,57 +271,56 @@ void ensure_device(pa_solver &s) {
 // scheduling
 // ---------------------------------------------------------------------------
 
-struct Plan {
-    std::vector<Prob> probs;    // in group order
-    std::vector<Group> groups;  // per launch, concatenated
-    std::vector<std::pair<int, std::pair<int, int>>> launches;  // R, [first group, end group)
-    int64_t rowbuf = 0;         // float2 elements
-    int64_t tb_words = 0;
-    int64_t dvals = 0;
-    double cells = 0.0;
-};
-
 int ilog2(int x) {
     int r = 0;
     while ((1 << r) < x) r++;
     return r;
 }
 
-// Choose (L, R) per problem: the most efficient layout whose per-lane
-// serial work stays under the cap that keeps >= 4 wavefronts of work per
-// SIMD; the largest problems fall back to the widest segment.
-void choose_layout(int64_t ls, int64_t lt, const std::vector<int> &Rs, double cap, int &L_out, int &R_out) {
-    double best_eff = -1.0, best_serial = 1e300;
+// Choose (L, R) per problem.  A lane-step costs STEP_OVERHEAD + R cell
+// units (the per-step shifts, symbol load and masking are worth about four
+// cells), so the layout cost is L * strips * (len_t + L - 1) * (4 + R) and a
+// segment's serial time is that over L.  Take the cheapest layout whose
+// serial time stays under `cap` (which keeps several wavefronts of work per
+// SIMD and the longest problem off the critical path); problems that fit no
+// layout take the one with the least serial time.
+constexpr double STEP_OVERHEAD = 4.0;
+
+void choose_layout(int64_t ls, int64_t lt, const std::vector<int> &Rs, double cap, int &L_out, int &R_out,
+                   double *cost_out = nullptr) {
+    double best_cost = 1e300, best_serial = 1e300;
     int bL = 64, bR = Rs.back();
-    double fallback_serial = 1e300;
+    double fb_serial = 1e300, fb_cost = 1e300;
     int fL = 64, fR = Rs.back();
     for (int L = 1; L <= 64; L <<= 1) {
         for (int R : Rs) {
             const int64_t rps = int64_t(L) * R;
             const int64_t nst = (ls + rps - 1) / rps;
-            const double steps = double(lt + L - 1);
-            const double serial = double(nst) * steps * R;
-            const double eff = double(ls) * double(lt) / (double(nst * rps) * steps);
-            if (serial < fallback_serial || (serial == fallback_serial && L > fL)) {
-                fallback_serial = serial;
+            const double serial = double(nst) * double(lt + L - 1) * (STEP_OVERHEAD + R);
+            const double cost = serial * L;
+            if (serial < fb_serial || (serial == fb_serial && cost < fb_cost)) {
+                fb_serial = serial;
+                fb_cost = cost;
                 fL = L;
                 fR = R;
             }
             if (serial > cap) continue;
-            if (eff > best_eff + 1e-9 || (std::fabs(eff - best_eff) <= 1e-9 && serial < best_serial)) {
-                best_eff = eff;
+            if (cost < best_cost || (cost == best_cost && serial < best_serial)) {
+                best_cost = cost;
                 best_serial = serial;
                 bL = L;
                 bR = R;
             }
         }
     }
-    if (best_eff < 0) {
+    if (best_cost == 1e300) {
         bL = fL;
         bR = fR;
+        best_cost = fb_cost;
     }
     L_out = bL;
     R_out = bR;
+    if (cost_out) *cost_out = best_cost;
 }
 
 struct Job {
@@ -304,18 +330,21 @@ struct Job {
 
 Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, int simds) {
     std::vector<int> Rs;
-    for (int R : {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32})
-        if (supported_R(job.form, job.tr, job.fast, R)) Rs.push_back(R);
-    if (job.tr == TR_ALL) Rs = {8};
-    // total work at the most efficient layout (no cap) sets the serial cap
+    if (job.fast) {
+        Rs.assign(std::begin(FAST_RS), std::end(FAST_RS));
+    } else {
+        for (int R : {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32})
+            if (supported_R(job.form, job.tr, job.fast, R)) Rs.push_back(R);
+    }
+    // total wavefront time at the cheapest layouts (no cap) sets the cap:
+    // four wavefronts' worth of work per SIMD
     double total = 0.0;
     for (int64_t k = 0; k < n; k++) {
         if (ls[k] <= 0 || lt[k] <= 0) continue;
         int L, R;
-        choose_layout(ls[k], lt[k], Rs, 1e300, L, R);
-        const int64_t rps = int64_t(L) * R;
-        const int64_t nst = (ls[k] + rps - 1) / rps;
-        total += double(nst) * rps * double(lt[k] + L - 1) / 64.0;
+        double cost = 0.0;
+        choose_layout(ls[k], lt[k], Rs, 1e300, L, R, &cost);
+        total += cost / 64.0;
     }
     const double cap = std::max(256.0, total / (double(simds) * 4.0));
 
@@ -368,10 +397,12 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
             p++;
         }
         g.count = cnt;
-        g.cost = int64_t(g.max_nst) * (g.max_lt + L - 1) * R;
-        gs.push_back(GroupR{g, R});
+        g.R = R;
+        g.cost = int64_t(double(g.max_nst) * double(g.max_lt + L - 1) * (STEP_OVERHEAD + R));
+        gs.push_back(GroupR{g, job.fast ? 0 : R});
     }
-    // one launch per R; inside, longest groups first (LPT)
+    // one launch per kernel (the fast kernel serves every R); inside a
+    // launch the longest groups go first (LPT)
     std::stable_sort(gs.begin(), gs.end(), [](const GroupR &a, const GroupR &b) {
         if (a.R != b.R) return a.R > b.R;
         return a.g.cost > b.g.cost;
@@ -423,6 +454,7 @@ Plan plan_plain(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n)
         g.lgL = 0;
         g.max_lt = int32_t(mt);
         g.max_nst = int32_t((ms + R - 1) / R);
+        g.R = R;
         g.cost = int64_t(g.max_nst) * mt * R;
         for (int64_t i : idx) {
             Prob pr{};
@@ -433,7 +465,8 @@ Plan plan_plain(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n)
         }
         plan.groups.push_back(g);
     }
-    plan.launches.push_back({R, {0, int(plan.groups.size())}});
+    // same kernel as the batched form: only the schedule differs
+    plan.launches.push_back({job.fast ? 0 : R, {0, int(plan.groups.size())}});
     return plan;
 }
 
@@ -506,43 +539,72 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
                o.gap != GAP_DTW && in.nan_free;
 
     hipStream_t st = s.stream;
-    Plan plan = in.schedule == PA_SCHED_PLAIN ? plan_plain(job, in.len_s, in.len_t, n)
-                                              : plan_auto(job, in.len_s, in.len_t, n, s.cus * 4);
-    // offsets: payload, traceback, row buffer, D values
+    // ---- schedule (reused while lengths, offsets and job repeat) ------------
     const int bits = tb_bits(job.gap, job.tr);
-    for (Prob &pr : plan.probs) {
-        const int64_t k = pr.out;
-        pr.len_s = in.len_s[k];
-        pr.len_t = in.len_t[k];
-        if (in.form == PA_FORM_DENSE) {
-            pr.off_s = in.dense_off ? in.dense_off[k] : k;
-            pr.off_t = 0;
-            pr.si = in.dense_si_fixed ? in.dense_si_fixed : pr.len_t;
-            pr.sj = in.dense_sj_fixed ? in.dense_sj_fixed : 1;
-        } else {
-            pr.off_s = in.off_s[k];
-            pr.off_t = in.off_t[k];
+    const int64_t *o_s = in.form == PA_FORM_DENSE ? in.dense_off : in.off_s;
+    const int64_t *o_t = in.form == PA_FORM_DENSE ? nullptr : in.off_t;
+    const std::vector<int64_t> key = {n, job.loc, job.dir, job.gap, job.form, job.tr, int64_t(job.fast),
+                                      in.schedule, in.dense_si_fixed, in.dense_sj_fixed, s.cus};
+    PlanCache &pc = s.last_plan;
+    auto same = [&]() {
+        if (!pc.valid || pc.key != key) return false;
+        if (std::memcmp(pc.len_s.data(), in.len_s, size_t(n) * 4) != 0) return false;
+        if (std::memcmp(pc.len_t.data(), in.len_t, size_t(n) * 4) != 0) return false;
+        if ((o_s == nullptr) != pc.off_s.empty() || (o_t == nullptr) != pc.off_t.empty()) return false;
+        if (o_s && std::memcmp(pc.off_s.data(), o_s, size_t(n) * 8) != 0) return false;
+        if (o_t && std::memcmp(pc.off_t.data(), o_t, size_t(n) * 8) != 0) return false;
+        return true;
+    };
+    const bool reuse = same();
+    if (!reuse) {
+        pc.valid = false;
+        pc.plan = in.schedule == PA_SCHED_PLAIN ? plan_plain(job, in.len_s, in.len_t, n)
+                                                : plan_auto(job, in.len_s, in.len_t, n, s.cus * 4);
+        Plan &pl = pc.plan;
+        // offsets: payload, traceback, row buffer, D values
+        for (Prob &pr : pl.probs) {
+            const int64_t k = pr.out;
+            pr.len_s = in.len_s[k];
+            pr.len_t = in.len_t[k];
+            if (in.form == PA_FORM_DENSE) {
+                pr.off_s = in.dense_off ? in.dense_off[k] : k;
+                pr.off_t = 0;
+                pr.si = in.dense_si_fixed ? in.dense_si_fixed : pr.len_t;
+                pr.sj = in.dense_sj_fixed ? in.dense_sj_fixed : 1;
+            } else {
+                pr.off_s = in.off_s[k];
+                pr.off_t = in.off_t[k];
+            }
+            const int64_t rps = int64_t(pr.L) * pr.R;
+            const int64_t nst = (pr.len_s > 0 && pr.len_t > 0) ? (pr.len_s + rps - 1) / rps : 0;
+            if (bits) {
+                pr.tb_off = pl.tb_words;
+                pl.tb_words += nst * (pr.len_t + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
+            }
+            if (nst > 1) {
+                pr.rb_off = pl.rowbuf;
+                pl.rowbuf += pr.len_t;
+            }
+            if (job.tr == TR_ALL && job.loc == LOC_LOCAL) {
+                pr.dv_off = pl.dvals;
+                pl.dvals += int64_t(pr.len_s) * pr.len_t;
+            }
+            pl.cells += double(pr.len_s) * double(pr.len_t);
         }
-        const int64_t rps = int64_t(pr.L) * pr.R;
-        const int64_t nst = (pr.len_s > 0 && pr.len_t > 0) ? (pr.len_s + rps - 1) / rps : 0;
-        if (bits) {
-            pr.tb_off = plan.tb_words;
-            plan.tb_words += nst * (pr.len_t + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
-        }
-        if (nst > 1) {
-            pr.rb_off = plan.rowbuf;
-            plan.rowbuf += pr.len_t;
-        }
-        if (job.tr == TR_ALL && job.loc == LOC_LOCAL) {
-            pr.dv_off = plan.dvals;
-            plan.dvals += int64_t(pr.len_s) * pr.len_t;
-        }
-        plan.cells += double(pr.len_s) * double(pr.len_t);
+        upload(st, s.b_probs, pl.probs.data(), pl.probs.size());
+        upload(st, s.b_groups, pl.groups.data(), pl.groups.size());
+        pc.key = key;
+        pc.len_s.assign(in.len_s, in.len_s + n);
+        pc.len_t.assign(in.len_t, in.len_t + n);
+        pc.off_s.clear();
+        if (o_s) pc.off_s.assign(o_s, o_s + n);
+        pc.off_t.clear();
+        if (o_t) pc.off_t.assign(o_t, o_t + n);
+        pc.valid = true;
     }
+    const Plan &plan = pc.plan;
 
     // ---- device buffers ----------------------------------------------------
-    upload(st, s.b_probs, plan.probs.data(), plan.probs.size());
-    upload(st, s.b_groups, plan.groups.data(), plan.groups.size());
     s.b_queue.ensure(std::max<size_t>(plan.launches.size(), 1) * 4);
     HIPCHK(launch_zero_u32(s.b_queue.as<uint32_t>(), int64_t(plan.launches.size()), st));
     const void *codes_s = in.codes_s, *codes_t = in.codes_t;
@@ -611,7 +673,15 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         spec.R = R;
         spec.fastmax = job.fast;
         spec.lds_bytes = job.form == F_TABLE ? size_t(in.K1) * in.K1 * 4 : 0;
-        const int waves_per_cu = std::max(1, occupancy_waves(spec));
+        const std::vector<int> okey = {spec.loc, spec.dir, spec.gap, spec.form, spec.tr, spec.R,
+                                       int(spec.fastmax), int(spec.lds_bytes)};
+        int waves_per_cu = 0;
+        for (auto &e : s.occ_cache)
+            if (e.first == okey) waves_per_cu = e.second;
+        if (!waves_per_cu) {
+            waves_per_cu = std::max(1, occupancy_waves(spec));
+            s.occ_cache.push_back({okey, waves_per_cu});
+        }
         const int64_t wg_needed = (int64_t(g1 - g0) + 3) / 4;
         spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg_needed, int64_t(s.cus) * std::max(1, waves_per_cu / 4))));
         KArgs la = a;

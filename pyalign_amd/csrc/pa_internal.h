@@ -42,7 +42,9 @@ struct Group {
     int32_t L, lgL;      // lanes per problem and its log2
     int32_t max_lt;      // over members: step loop bound is max_lt + L - 1
     int32_t max_nst;     // over members: strips of L*R rows
-    int64_t cost;        // scheduling cost (lane-cells)
+    int32_t R;           // rows per lane (kernels built for several R dispatch on it)
+    int32_t pad;
+    int64_t cost;        // scheduling cost
 };
 
 struct KArgs {
@@ -66,6 +68,13 @@ struct KArgs {
     uint32_t *tb;      // traceback words
     float *dvals;      // TR_ALL: D values per cell
 };
+
+// R = 0 selects the kernel that serves every strip height in FAST_RS from
+// one queue (the group carries its R).
+#ifndef PA_FAST_RS_LIST
+#define PA_FAST_RS_LIST 8, 16, 24, 32
+#endif
+constexpr int FAST_RS[] = {PA_FAST_RS_LIST};
 
 struct LaunchSpec {
     int loc, dir, gap, form, tr, R;

@@ -388,6 +388,142 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
     }
 }
 
+// ---------------------------------------------------------------------------
+// The score-only local maximisation path over a LDS table (the batched local
+// Gotoh / Smith-Waterman workload).  Same decomposition as solve_group with
+// the traceback and seed bookkeeping removed: every value is >= +0, so
+// v_max_f32 reproduces the strict select exactly (see the file comment),
+// the local borders are +0, and the optimum is a plain running maximum
+// (padded rows read the pad symbol, whose worst-valued entries keep them
+// below the optimum).  The column's symbol is loaded one step ahead.
+// ---------------------------------------------------------------------------
+template <int GAP, int R>
+__device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g, const float *__restrict__ tab,
+                                                 int lane) {
+    constexpr float NEG = -__builtin_inff();
+    const int L = g.L;
+    const int seg = lane >> g.lgL;
+    const int l = lane & (L - 1);
+    const bool has = seg < g.count;
+    int64_t off_s = 0, off_t = 0, rb_off = 0;
+    int ls = 0, lt = 0, out = 0;
+    if (has) {
+        const Prob &pr = a.probs[g.first + seg];
+        off_s = pr.off_s;
+        off_t = pr.off_t;
+        rb_off = pr.rb_off;
+        ls = pr.len_s;
+        lt = pr.len_t;
+        out = pr.out;
+    }
+    const int rps = L * R;
+    const int nst = (ls > 0 && lt > 0) ? (ls + rps - 1) / rps : 0;
+    const int steps = g.max_lt + L - 1;
+    const int K1 = a.K1;
+    const int PAD = K1 - 1;
+    const float go_s = a.go_s, ge_s = a.ge_s, go_t = a.go_t, ge_t = a.ge_t;
+    const uint8_t *__restrict__ cs = (const uint8_t *)a.codes_s;
+    const uint8_t *__restrict__ ct = (const uint8_t *)a.codes_t;
+    float best = NEG;
+
+    for (int st = 0; st < g.max_nst; ++st) {
+        const bool act = st < nst;
+        const int row0 = st * rps + l * R;
+        int rowsym[R];
+        float Dl[R], Ql[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = row0 + r;
+            rowsym[r] = ((act && i < ls) ? int(cs[off_s + i]) : PAD) * K1;
+            Dl[r] = 0.0f;  // local border D(i, -1) = +0 (solver.h:2021-2029)
+            Ql[r] = NEG;   // Q(i, -1)
+        }
+        float botD = 0.0f, botP = NEG;
+        float prevD = 0.0f;  // D(row0-1, -1) = +0
+        int tnext = 0;
+        if (act && l == 0 && lt > 0) tnext = int(ct[off_t]);
+
+        for (int s = 0; s < steps; ++s) {
+            const int j = s - l;
+            float inD = shr1(botD), inP = shr1(botP);
+            if (l == 0) {
+                if (st == 0) {
+                    inD = 0.0f;  // D(-1, j) = +0
+                    inP = NEG;
+                } else if (act && j < lt) {
+                    const float2 v = a.rowbuf[rb_off + j];
+                    inD = v.x;
+                    inP = v.y;
+                }
+            }
+            const float Ddiag = prevD;
+            prevD = inD;
+            const int tsym = tnext;
+            if (act && j + 1 >= 0 && j + 1 < lt) tnext = int(ct[off_t + j + 1]);
+            if (act && j >= 0 && j < lt) {
+                float Dd = Ddiag, Pup = inP;
+                float Gup = inD + go_s;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float S = tab[rowsym[r] + tsym];
+                    const float dg = Dd + S;
+                    float D;
+                    if constexpr (GAP == GAP_AFF) {
+                        const float P = fmaxf(Gup, Pup + ge_s);
+                        const float Q = fmaxf(Dl[r] + go_t, Ql[r] + ge_t);
+                        D = fmaxf(fmaxf(fmaxf(dg, 0.0f), P), Q);
+                        Ql[r] = Q;
+                        Pup = P;
+                    } else {
+                        D = fmaxf(fmaxf(fmaxf(dg, 0.0f), Gup), Dl[r] + go_t);
+                    }
+                    best = fmaxf(best, D);
+                    Dd = Dl[r];
+                    Dl[r] = D;
+                    Gup = D + go_s;
+                }
+                botD = Dl[R - 1];
+                botP = Pup;
+                if (l == L - 1 && st + 1 < nst) a.rowbuf[rb_off + j] = make_float2(botD, botP);
+            }
+        }
+        __threadfence_block();
+    }
+    for (int m = 1; m < L; m <<= 1) best = fmaxf(best, __shfl_xor(best, m, WAVE));
+    if (has && l == 0) a.scores[out] = (ls == 0 || lt == 0) ? NEG : best;
+}
+
+template <int GAP, int... RS>
+__global__ __launch_bounds__(256) void pa_fast_kernel(KArgs a) {
+    extern __shared__ float lds_table[];
+    const int n = a.K1 * a.K1;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) lds_table[k] = a.table[k];
+    __syncthreads();
+    const int lane = threadIdx.x & (WAVE - 1);
+    while (true) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(a.queue, 1u);
+        t = uint32_t(__shfl(int(t), 0, WAVE));
+        if (t >= uint32_t(a.n_groups)) break;
+        const Group &g = a.groups[t];
+        const int R = g.R;
+        ((R == RS ? (solve_group_fast<GAP, RS>(a, g, lds_table, lane), true) : false) || ...);
+    }
+}
+
+template <int GAP>
+static hipError_t go_fast(const LaunchSpec &s, const KArgs &a, hipStream_t st, int *occ_out) {
+    const void *fn = reinterpret_cast<const void *>(&pa_fast_kernel<GAP, PA_FAST_RS_LIST>);
+    if (occ_out) {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 256, s.lds_bytes) != hipSuccess) blocks = 1;
+        *occ_out = blocks * 4;
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL((pa_fast_kernel<GAP, PA_FAST_RS_LIST>), dim3(s.grid), dim3(256), s.lds_bytes, st, a);
+    return hipGetLastError();
+}
+
 template <int LOC, int DIR, int GAP, int FORM, int TR, int R, bool FAST, int SD>
 __global__ __launch_bounds__(256) void pa_solve_kernel(KArgs a) {
     extern __shared__ float lds_table[];
@@ -458,6 +594,7 @@ static hipError_t dispatch_lg(const LaunchSpec &s, const KArgs &a, hipStream_t s
         if (s.form == F_TABLE && s.tr == TR_SCORE) {
             if (s.fastmax) {
                 if constexpr (LOC == LOC_LOCAL && DIR == DIR_MAX) {
+                    if (s.R == 0) return go_fast<GAP>(s, a, st, occ_out);
                     switch (s.R) {
                         case 1: PA_GO(F_TABLE, TR_SCORE, 1, true, 0);
                         case 2: PA_GO(F_TABLE, TR_SCORE, 2, true, 0);
