@@ -144,6 +144,7 @@ typedef struct pa_ragged {
     float eq, ne;       /* binary */
     const float *dense; /* dense payload */
     const int64_t *dense_off;
+    int64_t dense_ld;   /* dense row stride (0: rows of len_t, packed) */
 } pa_ragged;
 
 /* Scheduling of a ragged batch. */
@@ -167,6 +168,29 @@ int pa_solver_sync(pa_solver *solver);
  * stream): total milliseconds and launch count. */
 int pa_solver_last_kernel_ms(const pa_solver *solver, double *ms, int32_t *launches,
                              double *cells);
+
+/*
+ * Differentiation (no reference counterpart: the reference has no
+ * gradients).  The optimal score is piecewise linear in the similarity
+ * values and the gap costs; on the optimal path
+ *   dScore/dS(i, j)       = 1 on the path's matched cells (every path cell
+ *                           for DTW, whose cost enters every cell),
+ *   dScore/d(gap open)    = -(number of gap runs)        (maximize; + when minimizing),
+ *   dScore/d(gap extend)  = -(sum over runs of length-1)  with w(k) = open + extend*(k-1),
+ *   DTW with series:        dD/dx_i = sum over path cells (i, j) of 2 (x_i - y_j).
+ * pa_forward_ragged runs the one-optimal forward pass on device-resident
+ * inputs, writes the scores to scores_device and keeps the traceback on the
+ * device in a tape; pa_backward walks every path once.  Gradient buffers
+ * are accumulated into (+=) and have the forward payload's layout; grad_gap
+ * is float32[n][2] (open, extend; linear gap costs use the open slot for
+ * d/d(gap per symbol)).  Pass NULL for gradients not wanted.  The tape
+ * borrows the forward's payload and its solver: both must outlive it.
+ */
+typedef struct pa_tape pa_tape;
+int pa_forward_ragged(pa_solver *solver, const pa_ragged *batch, float *scores_device, pa_tape **tape);
+int pa_backward(pa_tape *tape, const float *grad_scores_device, float *grad_dense_device,
+                float *grad_series_s_device, float *grad_series_t_device, float *grad_gap_device);
+void pa_tape_free(pa_tape *tape);
 
 /* ---- results (the tuple of per-lane objects, factory.h:699-808) ------ */
 

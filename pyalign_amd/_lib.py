@@ -37,6 +37,7 @@ class Ragged(ctypes.Structure):
         ("table", ctypes.c_void_p), ("K", ctypes.c_int32), ("dim", ctypes.c_int32),
         ("eq", ctypes.c_float), ("ne", ctypes.c_float),
         ("dense", ctypes.c_void_p), ("dense_off", ctypes.c_void_p),
+        ("dense_ld", ctypes.c_int64),
     ]
 
 
@@ -61,6 +62,9 @@ SIGNATURES = {
     "pa_solver_sync": (ctypes.c_int, [_P]),
     "pa_solver_last_kernel_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double),
                                                 ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),
+    "pa_forward_ragged": (ctypes.c_int, [_P, ctypes.POINTER(Ragged), _P, ctypes.POINTER(_P)]),
+    "pa_backward": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "pa_tape_free": (None, [_P]),
     "pa_result_count": (_I64, [_P]),
     "pa_result_scores": (ctypes.c_int, [_P, _P]),
     "pa_result_has_path": (ctypes.c_int, [_P, _I64]),

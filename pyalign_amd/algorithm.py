@@ -448,7 +448,7 @@ _FORMS = {"dense": _lib.PA_FORM_DENSE, "indexed": _lib.PA_FORM_INDEXED,
 
 
 def make_ragged(form, len_s, len_t, codes_s=None, codes_t=None, off_s=None, off_t=None, table=None,
-                eq=0.0, ne=0.0, dense=None, dense_off=None, dim=0, on_device=False):
+                eq=0.0, ne=0.0, dense=None, dense_off=None, dense_ld=0, dim=0, on_device=False):
     """Build a pa_ragged descriptor; returns (struct, arrays to keep alive).
 
     With on_device=True the payload arguments are device pointers (ints).
@@ -496,6 +496,7 @@ def make_ragged(form, len_s, len_t, codes_s=None, codes_t=None, off_s=None, off_
     elif form == "dense":
         rg.dense = payload(dense, np.float32)
         rg.dense_off = arr(dense_off, np.int64)
+        rg.dense_ld = int(dense_ld)
     return rg, keep
 
 

@@ -215,6 +215,18 @@ struct pa_result {
     std::vector<std::unique_ptr<IterState>> iters;
 };
 
+// A forward pass kept for differentiation: the schedule's problem
+// descriptors, the seeds and the traceback bits, moved out of the solver's
+// scratch buffers so later solves cannot overwrite them.
+struct pa_tape {
+    pa_solver *solver = nullptr;
+    int64_t n = 0, n_probs = 0;
+    int loc = 0, gap = 0, form = 0, dir = 0, dim = 0;
+    DevBuf probs, seeds, tb;
+    const float *dense = nullptr;
+    const float *xs = nullptr, *xt = nullptr;
+};
+
 namespace {
 
 void make_borders(pa_solver &s) {
@@ -502,7 +514,8 @@ void upload(hipStream_t st, DevBuf &buf, const T *src, size_t count) {
     if (count) HIPCHK(hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
 }
 
-void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, float *scores_device) {
+void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, float *scores_device,
+         pa_tape *tape = nullptr) {
     ensure_device(s);
     const pa_options &o = s.opt;
     const int64_t n = in.n;
@@ -535,6 +548,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     if (o.type == PA_TYPE_DTW && job.form != F_DENSE && job.form != F_SERIES)
         fail(PA_EINVAL, "DTW solvers take dense costs or series");
     if (job.form == F_SERIES && (in.dim < 1 || in.dim > 4)) fail(PA_EINVAL, "series dimension must be 1..4");
+    if (tape) job.tr = TR_ONE;  // differentiation walks the one-optimal path
     job.fast = job.form == F_TABLE && job.tr == TR_SCORE && job.loc == LOC_LOCAL && job.dir == DIR_MAX &&
                o.gap != GAP_DTW && in.nan_free;
 
@@ -701,6 +715,27 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     s.last_launches = launches;
     s.last_cells = plan.cells;
 
+    if (tape) {
+        tape->solver = &s;
+        tape->n = n;
+        tape->n_probs = int64_t(plan.probs.size());
+        tape->loc = job.loc;
+        tape->gap = job.gap;
+        tape->form = job.form;
+        tape->dir = job.dir;
+        tape->dim = in.dim;
+        tape->dense = dense;
+        tape->xs = static_cast<const float *>(codes_s);
+        tape->xt = static_cast<const float *>(codes_t);
+        std::swap(tape->probs.p, s.b_probs.p);
+        std::swap(tape->probs.cap, s.b_probs.cap);
+        std::swap(tape->seeds.p, s.b_seeds.p);
+        std::swap(tape->seeds.cap, s.b_seeds.cap);
+        std::swap(tape->tb.p, s.b_tb.p);
+        std::swap(tape->tb.cap, s.b_tb.cap);
+        s.last_plan.valid = false;  // its descriptors moved to the tape
+        return;
+    }
     if (!result) {
         return;  // device scores: caller synchronises
     }
@@ -1086,6 +1121,15 @@ static void ragged_input(const pa_solver *s, const pa_ragged *b, Input &in) {
             in.dense = b->dense;
             in.dense_elems = end;
             in.dense_off = b->dense_off;
+            if (b->dense_ld > 0) {
+                for (int64_t k = 0; k < b->n; k++) {
+                    if (b->len_t[k] > b->dense_ld) fail(PA_EINVAL, "dense row stride below len_t");
+                    end = std::max<int64_t>(end, b->dense_off[k] + int64_t(b->len_s[k]) * b->dense_ld);
+                }
+                in.dense_elems = end;
+                in.dense_si_fixed = int32_t(b->dense_ld);
+                in.dense_sj_fixed = 1;
+            }
             break;
         }
         default: fail(PA_EINVAL, "unknown problem form");
@@ -1142,6 +1186,50 @@ int pa_solver_last_kernel_ms(const pa_solver *s, double *ms, int32_t *launches, 
         if (cells) *cells = s->last_cells;
     });
 }
+
+// ---- differentiation -------------------------------------------------------
+
+int pa_forward_ragged(pa_solver *s, const pa_ragged *batch, float *scores_device, pa_tape **tape) {
+    return guard([&] {
+        if (!s || !batch || !scores_device || !tape) fail(PA_EINVAL, "null argument");
+        if ((batch->form == PA_FORM_DENSE || batch->form == PA_FORM_SERIES) && !batch->on_device)
+            fail(PA_EINVAL, "differentiation needs the dense or series payload resident on the device");
+        if (s->opt.type == PA_TYPE_ALIGNMENT && s->opt.gap != GAP_LIN && s->opt.gap != GAP_AFF)
+            fail(PA_EINVAL, "unsupported gap cost for differentiation");
+        Input in{};
+        ragged_input(s, batch, in);
+        auto t = std::make_unique<pa_tape>();
+        run(*s, in, nullptr, scores_device, t.get());
+        *tape = t.release();
+    });
+}
+
+int pa_backward(pa_tape *t, const float *grad_scores_device, float *grad_dense_device,
+                float *grad_series_s_device, float *grad_series_t_device, float *grad_gap_device) {
+    return guard([&] {
+        if (!t || !t->solver || !grad_scores_device) fail(PA_EINVAL, "null argument");
+        BwArgs a{};
+        a.probs = t->probs.as<Prob>();
+        a.n = t->n_probs;
+        a.tb = t->tb.as<uint32_t>();
+        a.seeds = t->seeds.as<int32_t>();
+        a.global = t->loc == LOC_GLOBAL ? 1 : 0;
+        a.gsign = t->dir == DIR_MAX ? -1.0f : 1.0f;
+        a.grad_scores = grad_scores_device;
+        a.grad_dense = t->form == F_DENSE ? grad_dense_device : nullptr;
+        a.xs = t->xs;
+        a.xt = t->xt;
+        a.grad_xs = t->form == F_SERIES ? grad_series_s_device : nullptr;
+        a.grad_xt = t->form == F_SERIES ? grad_series_t_device : nullptr;
+        a.grad_gap = t->gap == GAP_DTW ? nullptr : grad_gap_device;
+        const int form = (t->form == F_TABLE || t->form == F_BINARY) ? F_DENSE : t->form;
+        if (t->form == F_TABLE || t->form == F_BINARY) a.grad_dense = nullptr;  // no similarity input to differentiate
+        HIPCHK(launch_backward(t->gap, form, t->dim, a, t->solver->stream));
+        HIPCHK(hipStreamSynchronize(t->solver->stream));
+    });
+}
+
+void pa_tape_free(pa_tape *t) { delete t; }
 
 // ---- results ---------------------------------------------------------------
 

@@ -87,6 +87,22 @@ hipError_t launch_solve(const LaunchSpec &spec, const KArgs &args, hipStream_t s
 hipError_t launch_walk(int gap, const Prob *probs, int64_t n, const uint32_t *tb, const int32_t *seeds,
                        int32_t *path, const int64_t *path_off, int32_t *path_len, hipStream_t stream);
 hipError_t launch_zero_u32(uint32_t *p, int64_t n, hipStream_t stream);
+
+// Backward pass over one-optimal tracebacks (one thread per problem).
+struct BwArgs {
+    const Prob *probs;
+    int64_t n;
+    const uint32_t *tb;
+    const int32_t *seeds;
+    int global;          // count the border gap run that ends a global path
+    float gsign;         // d score / d gap cost: -1 maximising, +1 minimising
+    const float *grad_scores;
+    float *grad_dense;   // dense form, payload layout
+    const float *xs, *xt;
+    float *grad_xs, *grad_xt;
+    float *grad_gap;     // [2 * out]
+};
+hipError_t launch_backward(int gap, int form, int dim, const BwArgs &args, hipStream_t stream);
 bool supported_R(int form, int tr, bool fastmax, int R);
 int occupancy_waves(const LaunchSpec &spec);  // resident waves per CU for this instantiation
 

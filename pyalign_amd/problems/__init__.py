@@ -5,11 +5,12 @@ from .instance import (Form, Problem as _BaseProblem, MatrixProblem, IndexedMatr
 from .function import Function, Coalesced, Dict, Equality
 from .factory import (Problem, ProblemFactory, general, AlphabetProblem, ByteAlphabetEncoder,
                       AlphabetEncoder, AlphabetProblemFactory, alphabetic, IntegerEncoder,
-                      BinaryProblem, BinaryProblemFactory, binary)
+                      BinaryProblem, BinaryProblemFactory, binary, SeriesProblem, SeriesProblemFactory,
+                      series)
 
 __all__ = [
     "Form", "MatrixProblem", "IndexedMatrixProblem", "BinaryMatrixProblem", "ProblemBatch", "ProblemBag",
     "Function", "Coalesced", "Dict", "Equality", "Problem", "ProblemFactory", "general", "AlphabetProblem",
     "ByteAlphabetEncoder", "AlphabetEncoder", "AlphabetProblemFactory", "alphabetic", "IntegerEncoder",
-    "BinaryProblem", "BinaryProblemFactory", "binary",
+    "BinaryProblem", "BinaryProblemFactory", "binary", "SeriesProblem", "SeriesProblemFactory", "series",
 ]

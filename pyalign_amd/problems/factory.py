@@ -262,3 +262,48 @@ class BinaryProblemFactory:
 
 
 binary = BinaryProblemFactory
+
+
+class SeriesProblem(MatrixProblem):
+    """Elastic alignment of two vector series x[n, d], y[m, d] under the
+    squared Euclidean cost sum_k (x[i,k] - y[j,k])^2, accumulated over k in
+    order in float32 (no fused multiply-add)."""
+
+    def __init__(self, x, y, **kwargs):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        if x.ndim == 1:
+            x = x[:, None]
+        if y.ndim == 1:
+            y = y[:, None]
+        if x.shape[1] != y.shape[1]:
+            raise ValueError(f"series dimensions differ: {x.shape[1]} vs {y.shape[1]}")
+        super().__init__((x.shape[0], y.shape[0]), x, y, **kwargs)
+
+    @property
+    def dim(self):
+        return self._s.shape[1]
+
+    def build_matrix(self, out):
+        x, y = self._s, self._t
+        c = None
+        for k in range(x.shape[1]):
+            d = x[:, k][:, None] - y[:, k][None, :]
+            c = d * d if c is None else c + d * d
+        out[:, :] = c
+
+
+class SeriesProblemFactory:
+    """Problems over vector series with squared Euclidean costs (for DTW,
+    ``ElasticSolver``).  The cost is evaluated on the GPU from the series;
+    no dense matrix is built for bags."""
+
+    def __init__(self, direction="minimize", dtype=np.float32):
+        self._direction = direction
+        self._dtype = dtype
+
+    def new_problem(self, x, y):
+        return SeriesProblem(x, y, direction=self._direction, dtype=self._dtype)
+
+
+series = SeriesProblemFactory

@@ -22,7 +22,7 @@ import numpy as np
 from . import algorithm
 from .gaps import GapCost, ConstantGapCost
 from .problems import ProblemBag, Form
-from .problems.factory import AlphabetProblem, BinaryProblem
+from .problems.factory import SeriesProblem
 
 
 class Solution:
@@ -507,6 +507,13 @@ class Solver:
                 ct, ot, _ = _encode_bag(problems, 1)
                 payload = dict(codes_s=cs, codes_t=ct, off_s=os_, off_t=ot, eq=eq, ne=ne)
                 form = "binary"
+            elif all(isinstance(p, SeriesProblem) for p in problems) and 1 <= problems[0].dim <= 4 and \
+                    all(p.dim == problems[0].dim for p in problems):
+                xs = np.concatenate([p.s for p in problems]) if problems else np.zeros((0, 1), np.float32)
+                ys = np.concatenate([p.t for p in problems]) if problems else np.zeros((0, 1), np.float32)
+                payload = dict(codes_s=xs.ravel(), codes_t=ys.ravel(), off_s=_offsets(ls.astype(np.int64)),
+                               off_t=_offsets(lt.astype(np.int64)), dim=problems[0].dim)
+                form = "series"
             else:
                 sizes = ls.astype(np.int64) * lt.astype(np.int64)
                 dense_off = _offsets(sizes)

@@ -9,6 +9,8 @@ enumerations identical in content and order.
 
 import typing
 
+import zlib
+
 import numpy as np
 import pytest
 
@@ -162,7 +164,7 @@ def build_factory(form, rng, direction):
 @pytest.mark.parametrize("direction", ["maximize", "minimize"])
 @pytest.mark.parametrize("form", FORMS)
 def test_random_score_and_solution(loc, gap, direction, form):
-    rng = np.random.default_rng(abs(hash((loc, str(gap), direction, form))) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((loc, gap, direction, form)).encode()))
     alphabet, pf = build_factory(form, rng, direction)
     seqs, probs = random_bag(rng, pf, alphabet, 37, 0, 40)
     cls, oloc = SOLVERS[loc]
@@ -189,7 +191,7 @@ def test_random_score_and_solution(loc, gap, direction, form):
 @pytest.mark.parametrize("gap", [("linear", 1), ("affine", 3, 1), ("affine", 1, 1)], ids=str)
 @pytest.mark.parametrize("direction", ["maximize", "minimize"])
 def test_random_all_optimal_in_order(loc, gap, direction):
-    rng = np.random.default_rng(abs(hash((loc, str(gap), direction, "all"))) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(repr((loc, gap, direction, "all")).encode()))
     alphabet, pf = build_factory("alphabetic", rng, direction)
     seqs, probs = random_bag(rng, pf, alphabet, 25, 0, 9)
     cls, oloc = SOLVERS[loc]
@@ -303,3 +305,62 @@ def test_length_limits_raise():
     a = np.zeros((1, 9), np.uint32)
     with pytest.raises(RuntimeError, match="exceeds configured maximum length 8"):
         solver.solve_binary_for_score(a, a, 1.0, -1.0, np.array([[9], [9]]))
+
+
+# ------------------------------------------------------------------ DTW ---
+
+def _dtw_oracle(C, direction, goal, lims):
+    return O.OracleResult(O.make_config(solver_type=O.DTW, direction=direction, goal=goal,
+                                        lim_s=lims[0], lim_t=lims[1]), C)
+
+
+@pytest.mark.parametrize("direction", ["minimize", "maximize"])
+def test_dtw_dense_general_problems(direction):
+    rng = np.random.default_rng(31)
+    pf = pa.general(lambda x, y: float(np.float32(abs(x - y) * 0.5)), direction=direction)
+    seqs = [(rng.integers(0, 9, int(rng.integers(0, 30))), rng.integers(0, 9, int(rng.integers(0, 30))))
+            for _ in range(30)]
+    probs = [pf.new_problem(list(a), list(b)) for a, b in seqs]
+    odir = O.MAXIMIZE if direction == "maximize" else O.MINIMIZE
+    lims = (rounded(max(len(a) for a, _ in seqs)), rounded(max(len(b) for _, b in seqs)))
+    for codomain, goal in ((pa.Score, O.GOAL_SCORE), (pa.Solution, O.GOAL_ONE)):
+        res = pa.ElasticSolver(codomain=codomain).solve(probs)
+        for p, r in zip(probs, res):
+            ref = _dtw_oracle(p.matrix.astype(np.float32), odir, goal, lims)
+            score = r if codomain is pa.Score else r.score
+            assert np.float32(score).tobytes() == ref.score.tobytes(), (p.s, p.t)
+            if codomain is pa.Solution:
+                assert np.array_equal(np.asarray(r.path, dtype=np.int32), ref.path())
+                assert r.complexity.name == "DTW"
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3, 4])
+def test_dtw_series_form(dim):
+    rng = np.random.default_rng(40 + dim)
+    pf = pa.series(direction="minimize")
+    data = [(np.cumsum(rng.normal(0, 0.1, size=(int(rng.integers(1, 90)), dim)), 0),
+             np.cumsum(rng.normal(0, 0.1, size=(int(rng.integers(1, 90)), dim)), 0)) for _ in range(25)]
+    probs = [pf.new_problem(x, y) for x, y in data]
+    lims = (rounded(max(len(x) for x, _ in data)), rounded(max(len(y) for _, y in data)))
+    for codomain, goal in ((pa.Score, O.GOAL_SCORE), (pa.Solution, O.GOAL_ONE)):
+        res = pa.ElasticSolver(codomain=codomain).solve(probs)
+        for p, r in zip(probs, res):
+            ref = _dtw_oracle(p.matrix, O.MINIMIZE, goal, lims)
+            score = r if codomain is pa.Score else r.score
+            assert np.float32(score).tobytes() == ref.score.tobytes()
+            if codomain is pa.Solution:
+                assert np.array_equal(np.asarray(r.path, dtype=np.int32), ref.path())
+
+
+def test_dtw_series_long_multi_strip():
+    rng = np.random.default_rng(50)
+    pf = pa.series(direction="minimize")
+    data = [(np.cumsum(rng.normal(0, 0.1, size=(int(rng.integers(600, 1500)), 3)), 0),
+             np.cumsum(rng.normal(0, 0.1, size=(int(rng.integers(600, 1500)), 3)), 0)) for _ in range(4)]
+    probs = [pf.new_problem(x, y) for x, y in data]
+    lims = (rounded(max(len(x) for x, _ in data)), rounded(max(len(y) for _, y in data)))
+    res = pa.ElasticSolver(codomain=pa.Solution).solve(probs)
+    for p, r in zip(probs, res):
+        ref = _dtw_oracle(p.matrix, O.MINIMIZE, O.GOAL_ONE, lims)
+        assert np.float32(r.score).tobytes() == ref.score.tobytes()
+        assert np.array_equal(np.asarray(r.path, dtype=np.int32), ref.path())
