@@ -287,6 +287,11 @@ class SolutionIterator:
         return Solution(score, path, self._r.alignment(self._k, score), None, max_degree_1=False)
 
 
+def device_count():
+    """Number of HIP devices the library sees (0 without a GPU)."""
+    return int(lib().pa_device_count())
+
+
 def _i32(x):
     return np.ascontiguousarray(x, dtype=np.int32)
 
