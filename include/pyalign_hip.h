@@ -214,6 +214,24 @@ int64_t pa_result_iter_path_len(const pa_result *r, int64_t k);
 int pa_result_iter_path(const pa_result *r, int64_t k, int32_t *uv);
 void pa_result_free(pa_result *r);
 
+/* Solution matrices (Solution::values / traceback_as_matrix /
+ * traceback_as_edges, factory.h:251-262 and solver.h:2355-2471; read by
+ * pyalign/solve.py:44-57).  pa_solve_ragged_matrices solves the batch again
+ * keeping every layer's cell values and the traceback codes; the solver's
+ * count (one or all optimal) fixes the traceback kind as in the reference
+ * (traceback_1 / traceback_n).  Layers: 3 (D, P, Q) for affine gaps, else 1. */
+int pa_solve_ragged_matrices(pa_solver *solver, const pa_ragged *batch, pa_result **out);
+int32_t pa_result_layers(const pa_result *r); /* 0: no matrices kept */
+/* float32[layers][len_s+1][len_t+1], border row and column included */
+int pa_result_values(const pa_result *r, int64_t k, float *out);
+/* int32[layers][len_s+1][len_t+1][2]: each cell's first predecessor (u, v),
+ * the index type's minimum where there is none */
+int pa_result_traceback(const pa_result *r, int64_t k, int32_t *out);
+/* edges of one layer: int32[n][2][2] ((i-1, j-1), (u, v)) per predecessor,
+ * cells row by row; pa_result_edges_len returns n (-1 on error) */
+int64_t pa_result_edges_len(const pa_result *r, int64_t k, int32_t layer);
+int pa_result_edges(const pa_result *r, int64_t k, int32_t layer, int32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

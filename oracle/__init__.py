@@ -82,6 +82,10 @@ def lib():
             getattr(L, fn).argtypes = [ctypes.c_void_p, P(ctypes.c_int32)]
         L.oc_values.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_float)]
         L.oc_traceback.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_int32)]
+        L.oc_tb_edges_len.restype = ctypes.c_int64
+        L.oc_tb_edges_len.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        L.oc_tb_edges.restype = None
+        L.oc_tb_edges.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_int32)]
         L.oc_baseline_indexed_score.restype = ctypes.c_double
         L.oc_baseline_indexed_score.argtypes = [
             P(OcConfig), P(ctypes.c_uint8), P(ctypes.c_int64), P(ctypes.c_int64),
@@ -195,6 +199,18 @@ class OracleResult:
         out = np.empty((k, self.len_s + 1, self.len_t + 1, 2), dtype=np.int32)
         for layer in range(k):
             L.oc_traceback(self._h, layer, out[layer].ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        return out
+
+    def traceback_edges(self):
+        """Per layer: int32[n, 2, 2] of ((i-1, j-1), (u, v)) traceback entries."""
+        L = lib()
+        out = []
+        for layer in range(L.oc_layers(self._h)):
+            n = L.oc_tb_edges_len(self._h, layer)
+            e = np.empty((n, 2, 2), dtype=np.int32)
+            if n:
+                L.oc_tb_edges(self._h, layer, e.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+            out.append(e)
         return out
 
 

@@ -676,6 +676,39 @@ void oc_traceback(const oc_handle *h, int32_t layer, int32_t *out) {
         }
 }
 
+// Solution::traceback_as_edges (solver.h:2431-2480): per layer, cells row by
+// row (border included), each cell's traceback entries in order:
+// ((i-1, j-1), (u, v)).  traceback_1 cells always hold one entry.
+static void tb_edges(const oc_handle *h, int32_t layer, int32_t *out, int64_t *count) {
+    const auto &s = *h->solver;
+    int64_t n = 0;
+    auto emit = [&](int64_t i, int64_t j, int32_t u, int32_t v) {
+        if (out) {
+            out[4 * n] = int32_t(i - 1);
+            out[4 * n + 1] = int32_t(j - 1);
+            out[4 * n + 2] = u;
+            out[4 * n + 3] = v;
+        }
+        n++;
+    };
+    for (int64_t i = 0; i <= h->len_s; i++)
+        for (int64_t j = 0; j <= h->len_t; j++) {
+            const int64_t c = s.idx(layer, i - 1, j - 1);
+            if (s.tbu) emit(i, j, s.tbu[c], s.tbv[c]);
+            else if (s.all) for (const Pt &p : s.tbn[c]) emit(i, j, p.u, p.v);
+        }
+    *count = n;
+}
+int64_t oc_tb_edges_len(const oc_handle *h, int32_t layer) {
+    int64_t n = 0;
+    tb_edges(h, layer, nullptr, &n);
+    return n;
+}
+void oc_tb_edges(const oc_handle *h, int32_t layer, int32_t *out) {
+    int64_t n = 0;
+    tb_edges(h, layer, out, &n);
+}
+
 // ---------------------------------------------------------------------------
 // CPU baseline: the reference's batched score path (B lanes, Morton-free
 // greedy grouping done by the caller) over indexed sequences, on T threads.

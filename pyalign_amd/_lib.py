@@ -75,6 +75,12 @@ SIGNATURES = {
     "pa_result_iter_path_len": (_I64, [_P, _I64]),
     "pa_result_iter_path": (ctypes.c_int, [_P, _I64, _P]),
     "pa_result_free": (None, [_P]),
+    "pa_solve_ragged_matrices": (ctypes.c_int, [_P, ctypes.POINTER(Ragged), ctypes.POINTER(_P)]),
+    "pa_result_layers": (_I32, [_P]),
+    "pa_result_values": (ctypes.c_int, [_P, _I64, _P]),
+    "pa_result_traceback": (ctypes.c_int, [_P, _I64, _P]),
+    "pa_result_edges_len": (_I64, [_P, _I64, _I32]),
+    "pa_result_edges": (ctypes.c_int, [_P, _I64, _I32, _P]),
 }
 
 _lib = None

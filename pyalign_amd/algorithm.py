@@ -194,7 +194,7 @@ class Solution:
     def _materialised(self):
         if self._lazy is None:
             raise NotImplementedError(
-                "Solution.values / traceback matrices are not materialised by the MI355X solver")
+                "Solution matrices need the problem's data on the host (device-resident inputs keep none)")
         return self._lazy()
 
     @property
@@ -208,6 +208,26 @@ class Solution:
     @property
     def traceback_as_edges(self):
         return self._materialised()["traceback_as_edges"]
+
+
+class _Matrices:
+    """Solution.values / traceback_as_matrix / traceback_as_edges of one
+    problem, computed on first use by solving it again with every layer kept
+    (solver.h:2355-2471) and then cached.  Holds a private copy of the
+    problem's data: the caller's batch buffers are reused between batches."""
+
+    def __init__(self, solver, form, len_s, len_t, payload):
+        self._solver = solver
+        self._form = form
+        self._ls, self._lt = int(len_s), int(len_t)
+        self._payload = payload
+        self._data = None
+
+    def __call__(self):
+        if self._data is None:
+            self._data = self._solver._matrices(self._form, self._ls, self._lt, self._payload)
+            self._payload = None
+        return self._data
 
 
 class _Result:
@@ -276,15 +296,17 @@ class SolutionIterator:
     """SolutionIterator (factory.h:354-385); solutions carry no algorithm
     record, as in the reference's iterator path (solver.h:1881)."""
 
-    def __init__(self, result, k):
-        self._r, self._k = result, k
+    def __init__(self, result, k, lazy=None, max_degree_1=False):
+        self._r, self._k, self._lazy = result, k, lazy
+        self._max_degree_1 = max_degree_1
 
     def next(self):
         x = self._r.iter_next(self._k)
         if x is None:
             return None
         score, path = x
-        return Solution(score, path, self._r.alignment(self._k, score), None, max_degree_1=False)
+        return Solution(score, path, self._r.alignment(self._k, score), None,
+                        max_degree_1=self._max_degree_1, lazy=self._lazy)
 
 
 def device_count():
@@ -319,7 +341,9 @@ class Solver:
         return lib().pa_solver_batch_size(self._h)
 
     # ---- shared result conversion ------------------------------------
-    def _wrap(self, h, len_s, len_t, kind):
+    def _wrap(self, h, len_s, len_t, kind, one=None):
+        """one(k) -> (form, payload) copies problem k's data for its
+        Solution matrices (None: not available)."""
         n = lib().pa_result_count(h)
         r = _Result(h, n, self._index_dtype, len_s, len_t)
         scores = r.scores()
@@ -327,21 +351,55 @@ class Solver:
             return scores
         if kind == "alignment":
             return tuple(r.alignment(k, scores[k]) for k in range(n))
+
+        def lazy(k):
+            if one is None:
+                return None
+            form, payload = one(k)
+            return _Matrices(self, form, r.len_s[k], r.len_t[k], payload)
+
         if kind == "solution":
             alg = _algorithm_for(self._opts)
             out = []
             for k in range(n):
                 path = r.path(k)
                 if path is None:
-                    out.append(Solution(None, None, None, alg))
+                    out.append(Solution(None, None, None, alg, lazy=lazy(k)))
                 else:
-                    out.append(Solution(float(scores[k]), path, r.alignment(k, scores[k]), alg))
+                    out.append(Solution(float(scores[k]), path, r.alignment(k, scores[k]), alg, lazy=lazy(k)))
             return tuple(out)
         if kind == "alignment_iterator":
             return tuple(AlignmentIterator(r, k) for k in range(n))
         if kind == "solution_iterator":
-            return tuple(SolutionIterator(r, k) for k in range(n))
+            # traceback_n keeps every predecessor; DTW stays traceback_1
+            deg1 = self._opts.solver == Type.DTW or self._opts.count == Count.ONE
+            return tuple(SolutionIterator(r, k, lazy(k), deg1) for k in range(n))
         raise ValueError(kind)
+
+    def _matrices(self, form, ls, lt, payload):
+        """Solve one problem again with its DP matrices kept."""
+        rg, keep = make_ragged(form, [ls], [lt], **payload)
+        h = ctypes.c_void_p()
+        check(lib().pa_solve_ragged_matrices(self._h, ctypes.byref(rg), ctypes.byref(h)))
+        del keep
+        r = _Result(h, 1, self._index_dtype, [ls], [lt])
+        layers = lib().pa_result_layers(h)
+        values = np.empty((layers, ls + 1, lt + 1), dtype=np.float32)
+        check(lib().pa_result_values(h, 0, values.ctypes.data))
+        tb = np.empty((layers, ls + 1, lt + 1, 2), dtype=np.int32)
+        check(lib().pa_result_traceback(h, 0, tb.ctypes.data))
+        edges = []
+        for layer in range(layers):
+            m = lib().pa_result_edges_len(h, 0, layer)
+            if m < 0:
+                check(_lib.PA_EINVAL)
+            e = np.empty((m, 2, 2), dtype=np.int32)
+            if m:
+                check(lib().pa_result_edges(h, 0, layer, e.ctypes.data))
+            edges.append(e.astype(self._index_dtype))
+        del r
+        return {"values": values, "traceback_as_matrix": tb.astype(self._index_dtype),
+                "traceback_as_edges": edges}
 
     def _dense(self, sim, length, kind):
         sim = np.ascontiguousarray(sim, dtype=np.float32)
@@ -351,7 +409,12 @@ class Solver:
         h = ctypes.c_void_p()
         check(lib().pa_solve_dense(self._h, sim.ctypes.data, sim.shape[0], sim.shape[1], sim.shape[2],
                                    length.ctypes.data, ctypes.byref(h)))
-        return self._wrap(h, length[0], length[1], kind)
+
+        def one(k):
+            m = sim[:length[0, k], :length[1, k], k]
+            return "dense", dict(dense=m.ravel().copy(), dense_off=[0])
+
+        return self._wrap(h, length[0], length[1], kind, one)
 
     def _indexed(self, a, b, sim, length, kind):
         a = np.ascontiguousarray(a, dtype=np.uint32)
@@ -367,7 +430,17 @@ class Solver:
         check(lib().pa_solve_indexed(self._h, a.ctypes.data, a.shape[1], b.ctypes.data, b.shape[1], a.shape[0],
                                      sim.ctypes.data, sim.shape[0], sim.shape[1], length.ctypes.data,
                                      ctypes.byref(h)))
-        return self._wrap(h, length[0], length[1], kind)
+        table = None
+        if kind.startswith("solution"):
+            K = max(sim.shape)  # square table; codes stay below each side's size
+            table = np.zeros((K, K), dtype=np.float32)
+            table[:sim.shape[0], :sim.shape[1]] = sim
+
+        def one(k):
+            return "indexed", dict(codes_s=a[k, :length[0, k]].astype(np.uint8),
+                                   codes_t=b[k, :length[1, k]].astype(np.uint8), off_s=[0], off_t=[0], table=table)
+
+        return self._wrap(h, length[0], length[1], kind, one)
 
     def _binary(self, a, b, eq, ne, length, kind):
         a = np.ascontiguousarray(a, dtype=np.uint32)
@@ -378,7 +451,12 @@ class Solver:
         h = ctypes.c_void_p()
         check(lib().pa_solve_binary(self._h, a.ctypes.data, a.shape[1], b.ctypes.data, b.shape[1], a.shape[0],
                                     float(eq), float(ne), length.ctypes.data, ctypes.byref(h)))
-        return self._wrap(h, length[0], length[1], kind)
+
+        def one(k):
+            return "binary", dict(codes_s=a[k, :length[0, k]].copy(), codes_t=b[k, :length[1, k]].copy(),
+                                  off_s=[0], off_t=[0], eq=eq, ne=ne)
+
+        return self._wrap(h, length[0], length[1], kind, one)
 
     # ---- the reference's 15 entry points (pyalign.h:24-41) -----------
     def solve_for_score(self, sim, length):
@@ -438,7 +516,10 @@ class Solver:
         h = ctypes.c_void_p()
         check(lib().pa_solve_ragged(self._h, ctypes.byref(rg), schedule, ctypes.byref(h)))
         del keep
-        return self._wrap(h, len_s, len_t, kind)
+        one = None
+        if kind.startswith("solution") and not payload.get("on_device", False):
+            one = _ragged_one(form, len_s, len_t, payload)
+        return self._wrap(h, len_s, len_t, kind, one)
 
     def last_kernel_ms(self):
         ms = ctypes.c_double()
@@ -446,6 +527,40 @@ class Solver:
         cells = ctypes.c_double()
         check(lib().pa_solver_last_kernel_ms(self._h, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(cells)))
         return ms.value, launches.value, cells.value
+
+
+def _ragged_one(form, len_s, len_t, p):
+    """Problem k's data out of a ragged payload, as a one-problem payload."""
+    ls = np.asarray(len_s, dtype=np.int64)
+    lt = np.asarray(len_t, dtype=np.int64)
+
+    def one(k):
+        a, b = int(ls[k]), int(lt[k])
+        if form == "dense":
+            o = int(np.asarray(p["dense_off"], dtype=np.int64)[k])
+            ld = int(p.get("dense_ld", 0)) or b
+            flat = np.asarray(p["dense"], dtype=np.float32).ravel()
+            m = np.zeros((a, b), dtype=np.float32)
+            for i in range(a):
+                m[i] = flat[o + i * ld:o + i * ld + b]
+            return "dense", dict(dense=m.ravel(), dense_off=[0])
+        os_ = int(np.asarray(p["off_s"], dtype=np.int64)[k])
+        ot = int(np.asarray(p["off_t"], dtype=np.int64)[k])
+        if form == "series":
+            d = int(p["dim"])
+            xs = np.asarray(p["codes_s"], dtype=np.float32).ravel()
+            xt = np.asarray(p["codes_t"], dtype=np.float32).ravel()
+            return "series", dict(codes_s=xs[os_ * d:(os_ + a) * d].copy(), codes_t=xt[ot * d:(ot + b) * d].copy(),
+                                  off_s=[0], off_t=[0], dim=d)
+        cs, ct = np.asarray(p["codes_s"]).ravel(), np.asarray(p["codes_t"]).ravel()
+        q = dict(codes_s=cs[os_:os_ + a].copy(), codes_t=ct[ot:ot + b].copy(), off_s=[0], off_t=[0])
+        if form == "indexed":
+            q["table"] = p["table"]
+        else:
+            q["eq"], q["ne"] = p.get("eq", 0.0), p.get("ne", 0.0)
+        return form, q
+
+    return one
 
 
 _FORMS = {"dense": _lib.PA_FORM_DENSE, "indexed": _lib.PA_FORM_INDEXED,

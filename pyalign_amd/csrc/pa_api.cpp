@@ -208,9 +208,13 @@ struct pa_result {
     // one-optimal paths
     std::vector<int64_t> path_off;
     std::vector<int32_t> path_len, paths;
-    // all-optimal: traceback bits and local D values
+    // all-optimal: traceback bits and local D values; Solution matrices:
+    // the traceback bits and every layer's values, plus the borders
     std::vector<uint32_t> tb;
     std::vector<float> dvals;
+    bool matrices = false;
+    int layers = 1, tr = TR_SCORE;
+    std::vector<float> border_s, border_t;
     std::vector<Prob> probs;  // indexed by problem
     std::vector<std::unique_ptr<IterState>> iters;
 };
@@ -504,6 +508,7 @@ struct Input {
     int dim = 0;
     int schedule = PA_SCHED_AUTO;
     bool nan_free = true;
+    bool matrices = false;  // keep every layer's cell values (Solution matrices)
 };
 
 int goal_to_tr(int goal) { return goal == PA_GOAL_SCORE ? TR_SCORE : (goal == PA_GOAL_ONE ? TR_ONE : TR_ALL); }
@@ -549,6 +554,9 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         fail(PA_EINVAL, "DTW solvers take dense costs or series");
     if (job.form == F_SERIES && (in.dim < 1 || in.dim > 4)) fail(PA_EINVAL, "series dimension must be 1..4");
     if (tape) job.tr = TR_ONE;  // differentiation walks the one-optimal path
+    if (in.matrices && job.tr == TR_SCORE) job.tr = TR_ONE;
+    const int val_layers = in.matrices && job.gap == GAP_AFF ? 3 : 1;
+    const bool keep_vals = in.matrices || (job.tr == TR_ALL && job.loc == LOC_LOCAL);
     job.fast = job.form == F_TABLE && job.tr == TR_SCORE && job.loc == LOC_LOCAL && job.dir == DIR_MAX &&
                o.gap != GAP_DTW && in.nan_free;
 
@@ -558,7 +566,8 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     const int64_t *o_s = in.form == PA_FORM_DENSE ? in.dense_off : in.off_s;
     const int64_t *o_t = in.form == PA_FORM_DENSE ? nullptr : in.off_t;
     const std::vector<int64_t> key = {n, job.loc, job.dir, job.gap, job.form, job.tr, int64_t(job.fast),
-                                      in.schedule, in.dense_si_fixed, in.dense_sj_fixed, s.cus};
+                                      in.schedule, in.dense_si_fixed, in.dense_sj_fixed, s.cus,
+                                      int64_t(in.matrices)};
     PlanCache &pc = s.last_plan;
     auto same = [&]() {
         if (!pc.valid || pc.key != key) return false;
@@ -599,9 +608,9 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
                 pr.rb_off = pl.rowbuf;
                 pl.rowbuf += pr.len_t;
             }
-            if (job.tr == TR_ALL && job.loc == LOC_LOCAL) {
+            if (keep_vals) {
                 pr.dv_off = pl.dvals;
-                pl.dvals += int64_t(pr.len_s) * pr.len_t;
+                pl.dvals += int64_t(val_layers) * pr.len_s * pr.len_t;
             }
             pl.cells += double(pr.len_s) * double(pr.len_t);
         }
@@ -647,7 +656,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         seeds = s.b_seeds.as<int32_t>();
         s.b_tb.ensure(size_t(std::max<int64_t>(plan.tb_words, 1)) * 4);
     }
-    if (job.tr == TR_ALL) s.b_dvals.ensure(size_t(std::max<int64_t>(plan.dvals, 1)) * 4);
+    if (plan.dvals > 0) s.b_dvals.ensure(size_t(plan.dvals) * 4);
 
     KArgs a{};
     a.probs = s.b_probs.as<Prob>();
@@ -670,7 +679,8 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     a.scores = scores;
     a.seeds = seeds;
     a.tb = job.tr != TR_SCORE ? s.b_tb.as<uint32_t>() : nullptr;
-    a.dvals = job.tr == TR_ALL ? s.b_dvals.as<float>() : nullptr;
+    a.dvals = plan.dvals > 0 ? s.b_dvals.as<float>() : nullptr;
+    a.val_layers = val_layers;
 
     HIPCHK(hipEventRecord(s.ev0, st));
     int launches = 0;
@@ -769,7 +779,14 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         r->seeds.resize(size_t(n) * 2);
         HIPCHK(hipMemcpyAsync(r->seeds.data(), seeds, size_t(n) * 8, hipMemcpyDeviceToHost, st));
     }
-    if (job.tr == TR_ALL) {
+    r->matrices = in.matrices;
+    r->layers = val_layers;
+    r->tr = job.tr;
+    if (in.matrices) {
+        r->border_s.assign(s.border_s.begin(), s.border_s.end());
+        r->border_t.assign(s.border_t.begin(), s.border_t.end());
+    }
+    if (job.tr == TR_ALL || in.matrices) {
         r->tb.resize(size_t(plan.tb_words));
         if (!r->tb.empty())
             HIPCHK(hipMemcpyAsync(r->tb.data(), s.b_tb.p, r->tb.size() * 4, hipMemcpyDeviceToHost, st));
@@ -778,7 +795,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             HIPCHK(hipMemcpyAsync(r->dvals.data(), s.b_dvals.p, r->dvals.size() * 4, hipMemcpyDeviceToHost, st));
         r->probs.resize(static_cast<size_t>(n));
         for (const Prob &pr : plan.probs) r->probs[size_t(pr.out)] = pr;
-        r->iters.resize(static_cast<size_t>(n));
+        if (job.tr == TR_ALL) r->iters.resize(static_cast<size_t>(n));
     }
     if (n) HIPCHK(hipMemcpyAsync(r->scores.data(), scores, size_t(n) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1318,18 +1335,24 @@ int pa_result_alignment(const pa_result *r, int64_t k, int32_t *s_to_t, int32_t 
 
 // ---- all-optimal enumeration over the GPU's 8-bit tie sets ----------------
 
-static uint32_t tie_bits(const pa_result *r, const Prob &pr, int32_t i, int32_t j) {
+// the traceback code of cell (i, j): bits per cell 2 (linear, DTW), 4
+// (affine) or 8 (all-optimal tie sets); layout as written by the kernels
+static uint32_t cell_bits(const pa_result *r, const Prob &pr, int32_t i, int32_t j, int bits) {
     const int L = pr.L, R = pr.R;
     const int rps = L * R;
-    const int wpc = (8 * R + 31) / 32;
+    const int wpc = (bits * R + 31) / 32;
     const int st = i / rps;
     const int rem = i - st * rps;
     const int kk = rem / R;
     const int rr = rem - kk * R;
     const int64_t s = j + kk;
     const int64_t nsteps = pr.len_t + L - 1;
-    const uint32_t w = r->tb[size_t(pr.tb_off + ((int64_t(st) * nsteps + s) * L + kk) * wpc + (rr * 8) / 32)];
-    return (w >> ((rr * 8) % 32)) & 0xffu;
+    const uint32_t w = r->tb[size_t(pr.tb_off + ((int64_t(st) * nsteps + s) * L + kk) * wpc + (rr * bits) / 32)];
+    return (w >> ((rr * bits) % 32)) & ((1u << bits) - 1u);
+}
+
+static uint32_t tie_bits(const pa_result *r, const Prob &pr, int32_t i, int32_t j) {
+    return cell_bits(r, pr, i, j, 8);
 }
 
 // the edge list of a traceback_n cell: candidates equal to the cell's value
@@ -1462,5 +1485,197 @@ int pa_result_iter_path(const pa_result *r, int64_t k, int32_t *uv) {
 }
 
 void pa_result_free(pa_result *r) { delete r; }
+
+}  // extern "C"
+
+// ---- Solution matrices (Solution::set_values / traceback_as_matrix /
+// traceback_as_edges, solver.h:2355-2471) ------------------------------------
+//
+// Values: interior cells are the kernel's per-layer cell values; the border
+// row and column hold the solver's initial values (affine: D and P share the
+// column, D and Q the row, P's row and Q's column hold the worst value,
+// solver.h:2926-2951; DTW: worst except D(-1,-1) = 0, solver.h:3205-3210).
+// Tracebacks: border cells are cleared (solver.h:2953-2966); interior cells
+// are rebuilt from the traceback codes.  One optimal path (traceback_1,
+// solver.h:196-278): the single predecessor, which P and Q copy along
+// extended gaps (solver.h:2993-3016).  All optimal paths (traceback_n,
+// solver.h:280-360): the predecessors of every candidate equal to the cell's
+// value, in push order.
+
+static const Prob &matrices_prob(const pa_result *r, int64_t k, int32_t layer) {
+    if (!r || k < 0 || k >= r->n) fail(PA_EINVAL, "bad problem index");
+    if (!r->matrices) fail(PA_EINVAL, "the result keeps no solution matrices");
+    if (layer < 0 || layer >= r->layers) fail(PA_EINVAL, "bad layer");
+    return r->probs[size_t(k)];
+}
+
+static int32_t no_traceback_of(const pa_result *r) {  // solver.h:182-185
+    return r->index_bits == 16 ? int32_t(INT16_MIN) : INT32_MIN;
+}
+
+// first predecessor per cell and layer, [layers][ls+1][lt+1][2]
+static void fill_traceback(const pa_result *r, const Prob &pr, int32_t *out) {
+    const int64_t ls = pr.len_s, lt = pr.len_t, W = lt + 1, plane = (ls + 1) * W;
+    const int32_t no_tb = no_traceback_of(r);
+    for (int64_t q = 0; q < r->layers * plane * 2; q++) out[q] = no_tb;
+    const bool aff = r->gap == GAP_AFF;
+    const bool all = r->tr == TR_ALL;
+    const int bits = all ? 8 : (aff ? 4 : 2);
+    // all-optimal P/Q lists may be empty (a cleared border copied along)
+    std::vector<uint8_t> ne_p, ne_q;
+    if (all && aff) {
+        ne_p.assign(size_t(plane), 0);
+        ne_q.assign(size_t(plane), 0);
+    }
+    auto at = [&](int layer, int64_t i, int64_t j) { return out + 2 * (layer * plane + i * W + j); };
+    auto put = [](int32_t *c, int64_t u, int64_t v) {
+        c[0] = int32_t(u);
+        c[1] = int32_t(v);
+    };
+    for (int64_t i = 1; i <= ls; i++) {
+        for (int64_t j = 1; j <= lt; j++) {
+            const int64_t u = i - 1, v = j - 1;  // DP cell
+            const uint32_t b = cell_bits(r, pr, int32_t(u), int32_t(v), bits);
+            int32_t *d = at(0, i, j);
+            if (!all) {
+                if (aff) {
+                    int32_t *p = at(1, i, j), *q = at(2, i, j);
+                    if (b & 4u) std::memcpy(p, at(1, i - 1, j), 8);
+                    else put(p, u - 1, v);
+                    if (b & 8u) std::memcpy(q, at(2, i, j - 1), 8);
+                    else put(q, u, v - 1);
+                    switch (b & 3u) {
+                        case 0: break;  // local zero: (no_tb, no_tb)
+                        case 1: put(d, u - 1, v - 1); break;
+                        case 2: std::memcpy(d, p, 8); break;
+                        default: std::memcpy(d, q, 8); break;
+                    }
+                } else {
+                    switch (b & 3u) {
+                        case 0: break;
+                        case 1: put(d, u - 1, v - 1); break;
+                        case 2: put(d, u - 1, v); break;
+                        default: put(d, u, v - 1); break;
+                    }
+                }
+                continue;
+            }
+            bool p_ne = false, q_ne = false;
+            if (aff) {
+                int32_t *p = at(1, i, j), *q = at(2, i, j);
+                const int64_t c = i * W + j;
+                if (b & 16u) { put(p, u - 1, v); p_ne = true; }
+                else if ((b & 32u) && ne_p[size_t(c - W)]) { std::memcpy(p, at(1, i - 1, j), 8); p_ne = true; }
+                if (b & 64u) { put(q, u, v - 1); q_ne = true; }
+                else if ((b & 128u) && ne_q[size_t(c - 1)]) { std::memcpy(q, at(2, i, j - 1), 8); q_ne = true; }
+                ne_p[size_t(c)] = p_ne;
+                ne_q[size_t(c)] = q_ne;
+            }
+            if (b & 1u) continue;  // the local zero's (no_tb, no_tb) comes first
+            if (b & 2u) put(d, u - 1, v - 1);
+            else if (b & 4u) {
+                if (!aff) put(d, u - 1, v);
+                else if (p_ne) std::memcpy(d, at(1, i, j), 8);
+                else if ((b & 8u) && q_ne) std::memcpy(d, at(2, i, j), 8);
+            } else if (b & 8u) {
+                if (!aff) put(d, u, v - 1);
+                else if (q_ne) std::memcpy(d, at(2, i, j), 8);
+            }
+        }
+    }
+}
+
+// every traceback edge of one layer in the reference's order: cells row by
+// row, each cell's predecessors in push order; f(from_u, from_v, to_u, to_v)
+template <class F>
+static void for_each_edge(const pa_result *r, const Prob &pr, int32_t layer, F &&f) {
+    const int64_t ls = pr.len_s, lt = pr.len_t, W = lt + 1;
+    if (r->tr != TR_ALL) {
+        // traceback_1 cells always hold one entry (Single::size() == 1)
+        std::vector<int32_t> tb(size_t(r->layers * (ls + 1) * W * 2));
+        fill_traceback(r, pr, tb.data());
+        const int32_t *t = tb.data() + 2 * layer * (ls + 1) * W;
+        for (int64_t i = 0; i <= ls; i++)
+            for (int64_t j = 0; j <= lt; j++)
+                f(int32_t(i - 1), int32_t(j - 1), t[2 * (i * W + j)], t[2 * (i * W + j) + 1]);
+        return;
+    }
+    const int32_t no_tb = no_traceback_of(r);
+    std::vector<std::pair<int32_t, int32_t>> lst;
+    for (int64_t i = 1; i <= ls; i++) {  // border cells hold empty lists
+        for (int64_t j = 1; j <= lt; j++) {
+            const int32_t u = int32_t(i - 1), v = int32_t(j - 1);
+            lst.clear();
+            if (layer == 0) cell_edges(r, pr, u, v, no_tb, lst);
+            else push_gap_list(r, pr, u, v, layer == 1, lst);
+            for (const auto &e : lst) f(u, v, e.first, e.second);
+        }
+    }
+}
+
+extern "C" {
+
+int pa_solve_ragged_matrices(pa_solver *s, const pa_ragged *batch, pa_result **out) {
+    return guard([&] {
+        if (!s || !out) fail(PA_EINVAL, "null argument");
+        Input in{};
+        ragged_input(s, batch, in);
+        in.matrices = true;
+        std::unique_ptr<pa_result> r;
+        run(*s, in, &r, nullptr);
+        *out = r.release();
+    });
+}
+
+int32_t pa_result_layers(const pa_result *r) { return r && r->matrices ? r->layers : 0; }
+
+int pa_result_values(const pa_result *r, int64_t k, float *out) {
+    return guard([&] {
+        const Prob &pr = matrices_prob(r, k, 0);
+        if (!out) fail(PA_EINVAL, "null argument");
+        const int64_t ls = pr.len_s, lt = pr.len_t, W = lt + 1, plane = (ls + 1) * W;
+        const float wv = worst(r->dir);
+        for (int layer = 0; layer < r->layers; layer++) {
+            float *o = out + layer * plane;
+            for (int64_t j = 0; j <= lt; j++) o[j] = (layer == 1 && j > 0) ? wv : r->border_t[size_t(j)];
+            for (int64_t i = 1; i <= ls; i++) o[i * W] = layer == 2 ? wv : r->border_s[size_t(i)];
+            const float *src = r->dvals.data() + pr.dv_off + int64_t(layer) * ls * lt;
+            for (int64_t i = 1; i <= ls; i++)
+                std::memcpy(o + i * W + 1, src + (i - 1) * lt, size_t(lt) * 4);
+        }
+    });
+}
+
+int pa_result_traceback(const pa_result *r, int64_t k, int32_t *out) {
+    return guard([&] {
+        const Prob &pr = matrices_prob(r, k, 0);
+        if (!out) fail(PA_EINVAL, "null argument");
+        fill_traceback(r, pr, out);
+    });
+}
+
+int64_t pa_result_edges_len(const pa_result *r, int64_t k, int32_t layer) {
+    int64_t n = 0;
+    const int status = guard([&] {
+        const Prob &pr = matrices_prob(r, k, layer);
+        for_each_edge(r, pr, layer, [&](int32_t, int32_t, int32_t, int32_t) { n++; });
+    });
+    return status == PA_OK ? n : -1;
+}
+
+int pa_result_edges(const pa_result *r, int64_t k, int32_t layer, int32_t *out) {
+    return guard([&] {
+        const Prob &pr = matrices_prob(r, k, layer);
+        if (!out) fail(PA_EINVAL, "null argument");
+        int64_t q = 0;
+        for_each_edge(r, pr, layer, [&](int32_t a, int32_t b, int32_t c, int32_t d) {
+            out[4 * q] = a;
+            out[4 * q + 1] = b;
+            out[4 * q + 2] = c;
+            out[4 * q + 3] = d;
+            q++;
+        });
+    });
+}
 
 }  // extern "C"

@@ -27,7 +27,7 @@ struct Prob {
     int64_t off_t;   // codes / series rows of t
     int64_t tb_off;  // first 32-bit word of this problem's traceback
     int64_t rb_off;  // float2 offset of this problem's strip-boundary row buffer
-    int64_t dv_off;  // TR_ALL: offset of this problem's D values (row-major)
+    int64_t dv_off;  // offset of this problem's cell values: layer-major, then row-major
     int32_t len_s, len_t;
     int32_t si, sj;  // dense strides
     int32_t out;     // output slot (caller's problem index)
@@ -66,7 +66,8 @@ struct KArgs {
     float *scores;
     int32_t *seeds;    // [2*out]: seed (i, j), i = NO_SEED when there is none
     uint32_t *tb;      // traceback words
-    float *dvals;      // TR_ALL: D values per cell
+    float *dvals;      // cell values (null: none kept): D, then P and Q when val_layers == 3
+    int32_t val_layers;
 };
 
 // R = 0 selects the kernel that serves every strip height in FAST_RS from

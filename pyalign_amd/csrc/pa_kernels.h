@@ -279,8 +279,20 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                                 bi = i;
                                 bj = j;
                             }
-                            if constexpr (TR == TR_ALL) {
-                                if (rv[r]) a.dvals[pr.dv_off + int64_t(i) * lt + j] = D;
+                        }
+                    }
+                    // cell values: local all-optimal seeds read D; the Solution
+                    // matrices read every layer (solver.h:2355-2375)
+                    if constexpr (TR != TR_SCORE) {
+                        if (a.dvals != nullptr && rv[r]) {
+                            float *dv = a.dvals + pr.dv_off + int64_t(i) * lt + j;
+                            dv[0] = D;
+                            if constexpr (GAP == GAP_AFF) {
+                                if (a.val_layers == 3) {
+                                    const int64_t lay = int64_t(ls) * lt;
+                                    dv[lay] = P;
+                                    dv[2 * lay] = Q;
+                                }
                             }
                         }
                     }
