@@ -54,6 +54,100 @@ def c2_inputs(n=65536, seed=2):
     return table, ls, lt, codes_s, codes_t, off_s, off_t
 
 
+def c3_inputs(seed=3):
+    """One long pair: B uniform over ACGT (10,000), A a 5,000-long window of B
+    with 10% substitutions and indel events at p = 0.02 per position (half
+    insertions, half deletions, lengths Geometric(1/3))."""
+    rng = np.random.default_rng(seed)
+    alphabet = np.array(list("ACGT"))
+    B = rng.integers(0, 4, 10000)
+    o = int(rng.integers(0, 10000 - 5000 + 1))
+    window = B[o:o + 5000]
+    A = []
+    i = 0
+    while i < len(window):
+        x = int(window[i])
+        if rng.random() < 0.10:
+            x = (x + int(rng.integers(1, 4))) % 4
+        if rng.random() < 0.02:
+            k = int(rng.geometric(1 / 3))
+            if rng.random() < 0.5:  # insertion of k random bases after this one
+                A.append(x)
+                A.extend(rng.integers(0, 4, k).tolist())
+                i += 1
+            else:  # deletion of this and the next k - 1 positions
+                i += k
+            continue
+        A.append(x)
+        i += 1
+    return "".join(alphabet[A]), "".join(alphabet[B])
+
+
+def c3_bench(reps):
+    """Single long semiglobal affine pair, score-only and single Solution."""
+    import pyalign_amd as pa
+    a, b = c3_inputs()
+    pf = pa.binary(eq=1, ne=-1)
+    problem = pf.new_problem(a, b)
+    cells = float(len(a)) * len(b)
+    out = {"workload": f"single semiglobal affine pair (open 2.0, extend 0.5, binary eq 1 ne -1): "
+                       f"|A| = {len(a)}, |B| = {len(b)}", "cells": cells}
+    for name, codomain in (("score", pa.Score), ("solution", pa.Solution)):
+        solver = pa.SemiglobalSolver(gap_cost=pa.AffineGapCost(2.0, 0.5), codomain=codomain)
+        solver.solve(problem)  # warm: builds the solver and its buffers
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = solver.solve(problem)
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[name] = {"ms": round(ms, 3), "gcups": round(cells / (ms * 1e-3) / 1e9, 3)}
+        if name == "score":
+            out["optimal_score"] = float(r)
+    return out
+
+
+def c4_bench(steps):
+    """Differentiable global affine alignment: forward score (keeping the
+    traceback) against forward + backward (dScore/dS, d/d(open, extend))."""
+    import torch
+    from pyalign_amd.torch_ops import align_score
+    n, Lmax, dim = 1024, 512, 64
+    rng = np.random.default_rng(4)
+    ls = rng.integers(64, Lmax + 1, n).astype(np.int32)
+    lt = rng.integers(64, Lmax + 1, n).astype(np.int32)
+    g = torch.Generator(device="cpu").manual_seed(4)
+    A = (torch.randn(n, Lmax, dim, generator=g) / 8).cuda()
+    B = (torch.randn(n, Lmax, dim, generator=g) / 8).cuda()
+    S = torch.bmm(A, B.transpose(1, 2).contiguous()).contiguous()
+    cells = float((ls.astype(np.float64) * lt).sum())
+    go = torch.tensor(1.0, requires_grad=True)
+    ge = torch.tensor(0.2, requires_grad=True)
+    Sg = S.clone().requires_grad_(True)
+
+    def fwd():
+        with torch.no_grad():
+            align_score(S, ls, lt, 1.0, 0.2, locality="global")
+
+    def fwd_bwd():
+        Sg.grad = None
+        align_score(Sg, ls, lt, go, ge, locality="global").sum().backward()
+
+    res = {}
+    for name, fn in (("fwd", fwd), ("fwd_bwd", fwd_bwd)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) * 1e3 / steps
+    return {"workload": "differentiable global affine (open 1.0, extend 0.2): 1024 pairs, lengths U[64,512], "
+                        "S = dot products of N(0,1)/8 embeddings of dim 64 (float32, resident in HBM)",
+            "cells": cells, "fwd_ms": round(res["fwd"], 3), "fwd_bwd_ms": round(res["fwd_bwd"], 3),
+            "fwd_bwd_over_fwd": round(res["fwd_bwd"] / res["fwd"], 3),
+            "fwd_gcups": round(cells / (res["fwd"] * 1e-3) / 1e9, 2),
+            "fwd_bwd_gcups": round(cells / (res["fwd_bwd"] * 1e-3) / 1e9, 2)}
+
+
 def traffic():
     """HBM bytes per launch of the score kernel from the committed PMC runs
     (FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known size)."""
@@ -114,6 +208,7 @@ def main():
                     help="only run a float32 elementwise copy of this many bytes (PMC byte-counter scale)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the long-pair and fwd+bwd measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -203,6 +298,13 @@ def main():
                "scores_bitwise_equal": bool(np.array_equal(cpu_scores.view(np.uint32),
                                                            auto_scores[:sample].view(np.uint32)))}
 
+    long_pair = fwd_bwd = None
+    if not args.no_extra:
+        fwd_bwd = c4_bench(5)
+        log(f"fwd+bwd: {fwd_bwd['fwd_ms']:.2f} / {fwd_bwd['fwd_bwd_ms']:.2f} ms")
+        long_pair = c3_bench(2)
+        log(f"long pair: score {long_pair['score']['ms']:.1f} ms, solution {long_pair['solution']['ms']:.1f} ms")
+
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "GCUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
@@ -216,6 +318,8 @@ def main():
                      "ops_per_cell": OPS_PER_CELL, "kernel_ms_per_step": round(kms, 3)},
         "plain_gpu": plain,
         "cpu_baseline": cpu,
+        "fwd_bwd": fwd_bwd,
+        "long_pair": long_pair,
     }
     print(json.dumps(out))
 

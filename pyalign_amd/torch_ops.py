@@ -52,10 +52,25 @@ def _lengths(x, n):
     return a
 
 
+_SOLVERS = {}  # recent solvers by configuration (a tape keeps its own buffers)
+_MAX_SOLVERS = 8
+
+
 def _solver(kind, direction, locality, gap, gap_open, gap_extend, ls, lt):
     rounder = SizeRounder()
     lim_s = rounder(int(ls.max()) if len(ls) else 0)
     lim_t = rounder(int(lt.max()) if len(lt) else 0)
+    key = (kind, direction, locality, gap, gap_open, gap_extend, lim_s, lim_t)
+    solver = _SOLVERS.pop(key, None)
+    if solver is None:
+        solver = _new_solver(kind, direction, locality, gap, gap_open, gap_extend, lim_s, lim_t)
+    _SOLVERS[key] = solver  # most recent last
+    while len(_SOLVERS) > _MAX_SOLVERS:
+        _SOLVERS.pop(next(iter(_SOLVERS)))
+    return solver
+
+
+def _new_solver(kind, direction, locality, gap, gap_open, gap_extend, lim_s, lim_t):
     bits = algorithm.m16 if (lim_s < 16000 and lim_t < 16000) else algorithm.m32
     opts = dict(batch=True, return_dup=False, codomain=Codomain(Solution),
                 direction=algorithm.Direction.__members__[direction.upper()])
