@@ -139,36 +139,68 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         float prevD = W;
         if (l == 0) prevD = (row0 == 0) ? a.border_t[0] : (act ? a.border_s[row0] : W);
 
-        for (int s = 0; s < steps; ++s) {
+        // A step's global inputs are loaded one step ahead, so their latency
+        // overlaps the previous step's cells: the column's symbol (series
+        // row, or the similarity column for dense input) and, for lane 0,
+        // the row above the strip (border or row buffer).
+        int nx_sym = 0;
+        float nx_v[SDD];
+        float nx_S[FORM == F_DENSE ? R : 1];
+        float nx_D = W, nx_P = W;
+        auto fetch = [&](int s) {
             const int j = s - l;
-            float inD = shr1(botD), inP = shr1(botP);
+            const bool on = act && j >= 0 && j < lt;
+            if constexpr (FORM == F_TABLE) {
+                nx_sym = on ? int(((const uint8_t *)a.codes_t)[pr.off_t + j]) : 0;
+            } else if constexpr (FORM == F_BINARY) {
+                nx_sym = on ? int(((const uint32_t *)a.codes_t)[pr.off_t + j]) : 0;
+            } else if constexpr (FORM == F_DENSE) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    nx_S[r] = (on && rv[r]) ? a.dense[rowbase[r] + int64_t(j) * pr.sj] : W;
+            } else if constexpr (FORM == F_SERIES) {
+                const float *xt = (const float *)a.codes_t;
+#pragma unroll
+                for (int d = 0; d < SDD; ++d) nx_v[d] = on ? xt[(pr.off_t + j) * SD + d] : 0.0f;
+            }
             if (l == 0) {
-                if (st == 0) {
-                    inD = (act && j < lt) ? a.border_t[j + 1] : W;  // D(-1, j)
-                    inP = W;                                        // P(-1, j)
-                } else if (act && j < lt) {
-                    const float2 v = a.rowbuf[pr.rb_off + j];
-                    inD = v.x;
-                    inP = v.y;
+                nx_D = W;
+                nx_P = W;  // P(-1, j)
+                if (act && j < lt) {
+                    if (st == 0) {
+                        nx_D = a.border_t[j + 1];  // D(-1, j)
+                    } else {
+                        const float2 v = a.rowbuf[pr.rb_off + j];
+                        nx_D = v.x;
+                        nx_P = v.y;
+                    }
                 }
             }
+        };
+        fetch(0);
+
+        for (int s = 0; s < steps; ++s) {
+            const int j = s - l;
+            const int tsym = nx_sym;
+            [[maybe_unused]] float tv[SDD];
+            [[maybe_unused]] float Scol[FORM == F_DENSE ? R : 1];
+            if constexpr (FORM == F_SERIES) {
+#pragma unroll
+                for (int d = 0; d < SDD; ++d) tv[d] = nx_v[d];
+            }
+            if constexpr (FORM == F_DENSE) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) Scol[r] = nx_S[r];
+            }
+            float inD = shr1(botD), inP = shr1(botP);
+            if (l == 0) {
+                inD = nx_D;
+                inP = nx_P;
+            }
+            fetch(s + 1);
             const float Ddiag = prevD;
             prevD = inD;
             if (act && j >= 0 && j < lt) {
-                int tsym = 0;
-                int64_t coloff = 0;
-                float tv[SDD];
-                if constexpr (FORM == F_TABLE) {
-                    tsym = int(((const uint8_t *)a.codes_t)[pr.off_t + j]);
-                } else if constexpr (FORM == F_BINARY) {
-                    tsym = int(((const uint32_t *)a.codes_t)[pr.off_t + j]);
-                } else if constexpr (FORM == F_DENSE) {
-                    coloff = int64_t(j) * pr.sj;
-                } else if constexpr (FORM == F_SERIES) {
-                    const float *xt = (const float *)a.codes_t;
-#pragma unroll
-                    for (int d = 0; d < SDD; ++d) tv[d] = xt[(pr.off_t + j) * SD + d];
-                }
                 float Dd = Ddiag, Dup = inD, Pup = inP;
                 uint32_t tbw[WPC];
 #pragma unroll
@@ -183,7 +215,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                     } else if constexpr (FORM == F_BINARY) {
                         S = rv[r] ? (rowsym[r] == tsym ? a.eq : a.ne) : W;
                     } else if constexpr (FORM == F_DENSE) {
-                        S = rv[r] ? a.dense[rowbase[r] + coloff] : W;
+                        S = Scol[r];
                     } else {
                         // squared Euclidean cost, dimensions summed in order
                         float c = 0.0f;
@@ -519,7 +551,7 @@ __global__ __launch_bounds__(256) void pa_fast_kernel(KArgs a) {
         if (t >= uint32_t(a.n_groups)) break;
         const Group &g = a.groups[t];
         const int R = g.R;
-        ((R == RS ? (solve_group_fast<GAP, RS>(a, g, lds_table, lane), true) : false) || ...);
+        (void)((R == RS ? (solve_group_fast<GAP, RS>(a, g, lds_table, lane), true) : false) || ...);
     }
 }
 

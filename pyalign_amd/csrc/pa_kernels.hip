@@ -48,169 +48,313 @@ bool supported_R(int form, int tr, bool fastmax, int R) {
 }
 
 // ---------------------------------------------------------------------------
-// Traceback walk: one thread per problem follows the packed direction bits
-// from the seed (TracebackIterators<false>::Iterator::next, solver.h:1498-1536).
-// Affine gaps are single jumps to the gap's origin found through the P/Q
-// extend bits, because P and Q copy their predecessor's traceback
-// (solver.h:2999-3016).  Bit layout per problem: strip st, step s, lane k,
-// word w at ((st * (len_t + L - 1) + s) * L + k) * words + w, cell (i, j)
-// at lane k = (i mod L*R) / R, step s = j + k.
+// Traceback walks.  One wavefront follows one problem's path: the walk is
+// sequential, so the wave's 64 lanes do not split it; instead they load the
+// traceback words the walk is about to read, a tile at a time, into LDS
+// with coalesced loads, and every lane then steps through the tile in
+// lockstep (same state in every lane, made scalar with readfirstlane).  A
+// path only moves up and left, so a tile that ends at the current cell
+// (steps and lanes at or before it, same strip) holds the next stretch.
+// Bit layout per problem: strip st, step s, lane k, word w at
+// ((st * (len_t + L - 1) + s) * L + k) * words + w; cell (i, j) is lane
+// k = (i mod L*R) / R, row r = i mod R of that lane, step s = j + k.
 // ---------------------------------------------------------------------------
+// LDS per wavefront: a traceback tile of `tile_words` words (host-chosen:
+// small tiles and 4 wavefronts per block for many problems, one wavefront
+// with a large tile for a few long ones).
+struct WalkShape {
+    int waves;       // wavefronts (problems) per block
+    int tile_words;  // traceback words per wavefront tile
+};
+
+inline WalkShape walk_shape(int64_t n) {
+    return n >= 512 ? WalkShape{4, 2048} : WalkShape{1, 14336};
+}
+
+// A position (i, j) on one problem's traceback with its decomposition
+// (strip st, lane k, row r of the lane, step s = j + k), moved up and left
+// without divisions, and the LDS tile the codes are read from.
+struct TbWalker {
+    uint32_t *lds;
+    const uint32_t *tb;
+    int64_t tb_off, nsteps;
+    int L, R, wpc, bits, KT, ST;
+    int t_st, k0, k1;  // loaded tile: strip, lanes [k0, k1), steps [s0, s1)
+    int64_t s0, s1;
+    int i, j, st, k, r;
+    int64_t s;
+
+    __device__ void init(uint32_t *l, int tile_words, const uint32_t *t, const Prob &pr, int b) {
+        lds = l;
+        tb = t;
+        tb_off = pr.tb_off;
+        L = pr.L;
+        R = pr.R;
+        bits = b;
+        wpc = (b * R + 31) / 32;
+        nsteps = int64_t(pr.len_t) + L - 1;
+        KT = L < WAVE ? L : WAVE;
+        ST = tile_words / (KT * wpc);
+        t_st = -1;
+        k0 = k1 = 0;
+        s0 = s1 = 0;
+    }
+    __device__ void seek(int i_, int j_) {
+        i = i_;
+        j = j_;
+        const int rps = L * R;
+        st = i >= 0 ? i / rps : -1;
+        const int rem = i - st * rps;
+        k = rem / R;
+        r = rem - k * R;
+        s = int64_t(j) + k;
+    }
+    __device__ void up() {
+        --i;
+        if (--r < 0) {
+            r = R - 1;
+            --s;
+            if (--k < 0) {  // into the previous strip
+                k = L - 1;
+                s += L;
+                --st;
+            }
+        }
+    }
+    __device__ void left() {
+        --j;
+        --s;
+    }
+    // traceback code of the current cell (i, j >= 0; the same in every lane)
+    __device__ uint32_t code(int lane) {
+        if (st != t_st || k < k0 || s < s0) {
+            t_st = st;
+            k1 = k + 1;
+            k0 = k1 > KT ? k1 - KT : 0;
+            s1 = s + 1;
+            s0 = s1 > ST ? s1 - ST : 0;
+            const int kw = (k1 - k0) * wpc;
+            const int cnt = int(s1 - s0) * kw;
+            __builtin_amdgcn_wave_barrier();  // the old tile's reads are done
+            for (int w = lane; w < cnt; w += WAVE) {
+                const int si = w / kw;
+                const int q = w - si * kw;
+                lds[w] = tb[tb_off + ((int64_t(t_st) * nsteps + s0 + si) * L + k0) * wpc + q];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        const int bit = r * bits;
+        const uint32_t w = lds[(int(s - s0) * (k1 - k0) + (k - k0)) * wpc + (bit >> 5)];
+        return __builtin_amdgcn_readfirstlane((w >> (bit & 31)) & ((1u << bits) - 1u));
+    }
+};
+
+// One move along a one-optimal path (TracebackIterators<false>::Iterator
+// ::next, solver.h:1498-1536); returns the code's source (0: a local start,
+// D(i, j) not above zero, no move).  Affine gaps are single jumps to the
+// gap's origin found through the P/Q extend bits, because P and Q copy
+// their predecessor's traceback (solver.h:2999-3016).
+template <int GAP>
+__device__ __forceinline__ int walk_step(TbWalker &w, int lane) {
+    const int src = int(w.code(lane) & 3u);
+    if (src == 1) {
+        w.up();
+        w.left();
+    } else if (src == 2) {
+        if constexpr (GAP == GAP_AFF) {
+            while (w.i > 0 && (w.code(lane) & 4u)) w.up();
+        }
+        w.up();
+    } else if (src == 3) {
+        if constexpr (GAP == GAP_AFF) {
+            while (w.j > 0 && (w.code(lane) & 8u)) w.left();
+        }
+        w.left();
+    }
+    return src;
+}
+
 template <int GAP>
 __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t n, const uint32_t *tb,
                                                       const int32_t *seeds, int32_t *path,
-                                                      const int64_t *path_off, int32_t *path_len) {
-    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+                                                      const int64_t *path_off, int32_t *path_len,
+                                                      int tile_words) {
+    extern __shared__ uint32_t smem[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int64_t k = int64_t(blockIdx.x) * (blockDim.x / WAVE) + wave;
+    if (k >= n) return;  // whole wavefronts leave; no block-wide barriers follow
     const Prob pr = probs[k];
     const int out = pr.out;
-    int u = seeds[2 * out], v = seeds[2 * out + 1];
+    const int u = seeds[2 * out], v = seeds[2 * out + 1];
     if (u == NO_SEED) {
-        path_len[out] = -1;
+        if (lane == 0) path_len[out] = -1;
         return;
     }
-    constexpr int BITS = GAP == GAP_AFF ? 4 : 2;
-    const int L = pr.L, R = pr.R, lt = pr.len_t;
-    const int rps = L * R;
-    const int wpc = (BITS * R + 31) / 32;
-    const int64_t nsteps = lt + L - 1;
-    auto cell = [&](int i, int j) -> uint32_t {
-        const int st = i / rps;
-        const int rem = i - st * rps;
-        const int kk = rem / R;
-        const int r = rem - kk * R;
-        const int64_t s = j + kk;
-        const uint32_t w = tb[pr.tb_off + ((st * nsteps + s) * L + kk) * wpc + (r * BITS) / 32];
-        return (w >> ((r * BITS) % 32)) & ((1u << BITS) - 1u);
-    };
+    TbWalker w;
+    w.init(smem + wave * tile_words, tile_words, tb, pr, GAP == GAP_AFF ? 4 : 2);
+    w.seek(u, v);
     int32_t *P = path + 2 * path_off[out];
-    P[0] = u;
-    P[1] = v;
+    if (lane == 0) {
+        P[0] = u;
+        P[1] = v;
+    }
     int cnt = 1;
-    while (u >= 0 && v >= 0) {
-        const uint32_t c = cell(u, v);
-        const uint32_t src = c & 3u;
-        if (src == 0) break;  // local: D(u, v) is not above zero
-        if (src == 1) {
-            --u;
-            --v;
-        } else if (src == 2) {
-            if constexpr (GAP == GAP_AFF) {
-                int kk = u;
-                while (kk > 0 && (cell(kk, v) & 4u)) --kk;
-                u = kk - 1;
-            } else {
-                --u;
-            }
-        } else {
-            if constexpr (GAP == GAP_AFF) {
-                int kk = v;
-                while (kk > 0 && (cell(u, kk) & 8u)) --kk;
-                v = kk - 1;
-            } else {
-                --v;
-            }
+    while (w.i >= 0 && w.j >= 0) {
+        if (walk_step<GAP>(w, lane) == 0) break;
+        if (lane == 0) {
+            P[2 * cnt] = w.i;
+            P[2 * cnt + 1] = w.j;
         }
-        P[2 * cnt] = u;
-        P[2 * cnt + 1] = v;
         ++cnt;
     }
-    path_len[out] = cnt;
+    if (lane == 0) path_len[out] = cnt;
 }
 
 hipError_t launch_walk(int gap, const Prob *probs, int64_t n, const uint32_t *tb, const int32_t *seeds,
                        int32_t *path, const int64_t *path_off, int32_t *path_len, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    const dim3 grid(unsigned((n + 255) / 256)), block(256);
+    const WalkShape ws = walk_shape(n);
+    const dim3 grid(unsigned((n + ws.waves - 1) / ws.waves)), block(ws.waves * WAVE);
+    const size_t lds = size_t(ws.waves) * ws.tile_words * 4;
     if (gap == GAP_AFF)
-        hipLaunchKernelGGL(pa_walk_kernel<GAP_AFF>, grid, block, 0, st, probs, n, tb, seeds, path, path_off,
-                           path_len);
+        hipLaunchKernelGGL(pa_walk_kernel<GAP_AFF>, grid, block, lds, st, probs, n, tb, seeds, path, path_off,
+                           path_len, ws.tile_words);
     else  // linear and DTW share the 2-bit moves
-        hipLaunchKernelGGL(pa_walk_kernel<GAP_LIN>, grid, block, 0, st, probs, n, tb, seeds, path, path_off,
-                           path_len);
+        hipLaunchKernelGGL(pa_walk_kernel<GAP_LIN>, grid, block, lds, st, probs, n, tb, seeds, path, path_off,
+                           path_len, ws.tile_words);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Backward: one thread per problem walks its optimal path (same moves as
+// Backward: one wavefront per problem walks its optimal path (the moves of
 // pa_walk_kernel) and emits the gradients of the optimal score.  The score
 // is a sum along the path: S(u, v) for every cell left by a diagonal move
 // (the alignment's matched cells; path_compressor, solver.h:831-846), the
 // cost of every path cell for DTW (solver.h:3236-3248), and -w(k) per gap
 // run with w(k) = open + extend*(k - 1) (pyalign/gaps.py:164-176), including
 // the border run that ends a global path at (u, -1) or (-1, v) (the border
-// values w(u+1), solver.h:2943-2951).  Every problem writes only its own
-// region, so no atomics are needed.
+// values w(u+1), solver.h:2943-2951).  Each gradient entry is touched once
+// per call (path cells are distinct; a DTW path visits each row and each
+// column in one contiguous run, summed in registers in path order), so the
+// adds go out as no-return atomics and the walk never waits on them.  The
+// series rows the DTW cost needs come through an LDS window that, like the
+// traceback tile, ends at the current row and is refilled when left.
 // ---------------------------------------------------------------------------
+constexpr int SERIES_WIN = 256;  // rows per series window (x and y each)
+
+// float add to global memory with no value returned (nothing waits on it)
+__device__ __forceinline__ void add_noret(float *p, float v) { (void)unsafeAtomicAdd(p, v); }
+
+template <int SD>
+struct RowWin {
+    float *lds;        // SERIES_WIN * SD floats
+    const float *src;  // series rows of this problem
+    int lo, hi;        // rows [lo, hi) held
+
+    __device__ void init(float *l, const float *s) {
+        lds = l;
+        src = s;
+        lo = hi = 0;
+    }
+    __device__ const float *row(int i, int lane) {
+        if (i < lo || i >= hi) {
+            hi = i + 1;
+            lo = hi > SERIES_WIN ? hi - SERIES_WIN : 0;
+            __builtin_amdgcn_wave_barrier();
+            for (int w = lane; w < (hi - lo) * SD; w += WAVE) lds[w] = src[int64_t(lo) * SD + w];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        return lds + (i - lo) * SD;
+    }
+};
+
 template <int GAP, int FORM, int SD>
-__global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a) {
-    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_words) {
+    constexpr int SDD = SD > 0 ? SD : 1;
+    constexpr int WIN = FORM == F_SERIES ? 2 * SERIES_WIN * SDD : 0;  // floats per wavefront
+    extern __shared__ uint32_t smem[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int64_t k = int64_t(blockIdx.x) * (blockDim.x / WAVE) + wave;
     if (k >= a.n) return;
     const Prob pr = a.probs[k];
     const int out = pr.out;
     const float g = a.grad_scores[out];
-    int u = a.seeds[2 * out], v = a.seeds[2 * out + 1];
+    const int u0 = a.seeds[2 * out], v0 = a.seeds[2 * out + 1];
     float opens = 0.0f, extends = 0.0f;
-    constexpr int BITS = GAP == GAP_AFF ? 4 : 2;
-    const int L = pr.L, R = pr.R, lt = pr.len_t;
-    const int rps = L * R;
-    const int wpc = (BITS * R + 31) / 32;
-    const int64_t nsteps = lt + L - 1;
-    auto cell = [&](int i, int j) -> uint32_t {
-        const int st = i / rps;
-        const int rem = i - st * rps;
-        const int kk = rem / R;
-        const int r = rem - kk * R;
-        const int64_t s = j + kk;
-        const uint32_t w = a.tb[pr.tb_off + ((st * nsteps + s) * L + kk) * wpc + (r * BITS) / 32];
-        return (w >> ((r * BITS) % 32)) & ((1u << BITS) - 1u);
+    uint32_t *mine = smem + size_t(wave) * (tile_words + WIN);
+    TbWalker w;
+    w.init(mine, tile_words, a.tb, pr, GAP == GAP_AFF ? 4 : 2);
+    [[maybe_unused]] RowWin<SDD> wx, wy;
+    // DTW series: the running sums of the current row / column
+    [[maybe_unused]] float gx[SDD], gy[SDD];
+    [[maybe_unused]] int cur_i = -1, cur_j = -1;
+    if constexpr (FORM == F_SERIES) {
+        float *win = reinterpret_cast<float *>(mine + tile_words);
+        wx.init(win, a.xs + pr.off_s * SD);
+        wy.init(win + SERIES_WIN * SDD, a.xt + pr.off_t * SD);
+    }
+    auto flush_row = [&]() {
+        if constexpr (FORM == F_SERIES) {
+            if (cur_i >= 0 && a.grad_xs && lane == 0)
+                for (int d = 0; d < SD; ++d) add_noret(a.grad_xs + (pr.off_s + cur_i) * SD + d, gx[d]);
+        }
+    };
+    auto flush_col = [&]() {
+        if constexpr (FORM == F_SERIES) {
+            if (cur_j >= 0 && a.grad_xt && lane == 0)
+                for (int d = 0; d < SD; ++d) add_noret(a.grad_xt + (pr.off_t + cur_j) * SD + d, gy[d]);
+        }
     };
     auto cost_grad = [&](int i, int j) {
         if constexpr (FORM == F_DENSE) {
-            if (a.grad_dense) a.grad_dense[pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj] += g;
+            if (a.grad_dense && lane == 0)
+                add_noret(a.grad_dense + pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj, g);
         } else if constexpr (FORM == F_SERIES) {
+            if (i != cur_i) {
+                flush_row();
+                cur_i = i;
+                for (int d = 0; d < SD; ++d) gx[d] = 0.0f;
+            }
+            if (j != cur_j) {
+                flush_col();
+                cur_j = j;
+                for (int d = 0; d < SD; ++d) gy[d] = 0.0f;
+            }
+            const float *xr = wx.row(i, lane);
+            const float *yr = wy.row(j, lane);
 #pragma unroll
             for (int d = 0; d < SD; ++d) {
-                const float diff = a.xs[(pr.off_s + i) * SD + d] - a.xt[(pr.off_t + j) * SD + d];
-                const float gd = g * (2.0f * diff);
-                if (a.grad_xs) a.grad_xs[(pr.off_s + i) * SD + d] += gd;
-                if (a.grad_xt) a.grad_xt[(pr.off_t + j) * SD + d] -= gd;
+                const float gd = g * (2.0f * (xr[d] - yr[d]));
+                gx[d] += gd;
+                gy[d] -= gd;
             }
         }
     };
-    if (u != NO_SEED) {
-        while (u >= 0 && v >= 0) {
-            const uint32_t src = cell(u, v) & 3u;
-            if constexpr (GAP == GAP_DTW) cost_grad(u, v);
+    int u = u0, v = v0;
+    if (u0 != NO_SEED) {
+        w.seek(u0, v0);
+        while (w.i >= 0 && w.j >= 0) {
+            const int pu = w.i, pv = w.j;
+            const int src = walk_step<GAP>(w, lane);
+            if constexpr (GAP == GAP_DTW) cost_grad(pu, pv);
             if (src == 0) break;  // local: the path starts at a zero cell
             if (src == 1) {
-                if constexpr (GAP != GAP_DTW) cost_grad(u, v);
-                --u;
-                --v;
-            } else if (src == 2) {
-                if constexpr (GAP == GAP_AFF) {
-                    int kk = u;
-                    while (kk > 0 && (cell(kk, v) & 4u)) --kk;
-                    opens += 1.0f;
-                    extends += float(u - kk);
-                    u = kk - 1;
-                } else {
-                    opens += 1.0f;  // linear: one gap symbol
-                    --u;
-                }
-            } else {
-                if constexpr (GAP == GAP_AFF) {
-                    int kk = v;
-                    while (kk > 0 && (cell(u, kk) & 8u)) --kk;
-                    opens += 1.0f;
-                    extends += float(v - kk);
-                    v = kk - 1;
-                } else {
-                    opens += 1.0f;
-                    --v;
-                }
+                if constexpr (GAP != GAP_DTW) cost_grad(pu, pv);
+            } else if constexpr (GAP == GAP_AFF) {
+                opens += 1.0f;
+                extends += float((pu - w.i) + (pv - w.j) - 1);  // run length - 1
+            } else if constexpr (GAP == GAP_LIN) {
+                opens += 1.0f;  // linear: one gap symbol per move
             }
         }
+        u = w.i;
+        v = w.j;
+        flush_row();
+        flush_col();
         if (GAP != GAP_DTW && a.global) {
             const int run = (u >= 0 && v < 0) ? u + 1 : ((v >= 0 && u < 0) ? v + 1 : 0);
             if (run > 0) {
@@ -223,16 +367,20 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a) {
             }
         }
     }
-    if (a.grad_gap) {
-        a.grad_gap[2 * out] += a.gsign * g * opens;
-        a.grad_gap[2 * out + 1] += a.gsign * g * extends;
+    if (a.grad_gap && lane == 0) {
+        add_noret(a.grad_gap + 2 * out, a.gsign * g * opens);
+        add_noret(a.grad_gap + 2 * out + 1, a.gsign * g * extends);
     }
 }
 
 hipError_t launch_backward(int gap, int form, int dim, const BwArgs &args, hipStream_t st) {
     if (args.n == 0) return hipSuccess;
-    const dim3 grid(unsigned((args.n + 255) / 256)), block(256);
-#define PA_BW(G, F, D) hipLaunchKernelGGL((pa_backward_kernel<G, F, D>), grid, block, 0, st, args)
+    const WalkShape ws = walk_shape(args.n);
+    const dim3 grid(unsigned((args.n + ws.waves - 1) / ws.waves)), block(ws.waves * WAVE);
+    const int win = form == F_SERIES ? 2 * SERIES_WIN * dim : 0;
+    const size_t lds = size_t(ws.waves) * (ws.tile_words + win) * 4;
+#define PA_BW(G, F, D) \
+    hipLaunchKernelGGL((pa_backward_kernel<G, F, D>), grid, block, lds, st, args, ws.tile_words)
     if (form == F_DENSE) {
         if (gap == GAP_AFF) PA_BW(GAP_AFF, F_DENSE, 0);
         else if (gap == GAP_LIN) PA_BW(GAP_LIN, F_DENSE, 0);
