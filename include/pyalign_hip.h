@@ -168,6 +168,10 @@ int pa_solver_sync(pa_solver *solver);
  * stream): total milliseconds and launch count. */
 int pa_solver_last_kernel_ms(const pa_solver *solver, double *ms, int32_t *launches,
                              double *cells);
+/* Schedule of the last solve: work groups queued and problems run in band
+ * mode (a long problem's strips computed concurrently by several
+ * wavefronts).  No reference counterpart (diagnostics). */
+int pa_solver_last_schedule(const pa_solver *solver, int32_t *groups, int32_t *band_problems);
 
 /*
  * Differentiation (no reference counterpart: the reference has no

@@ -521,6 +521,13 @@ class Solver:
             one = _ragged_one(form, len_s, len_t, payload)
         return self._wrap(h, len_s, len_t, kind, one)
 
+    def last_schedule(self):
+        """(work groups, problems in band mode) of the last solve."""
+        groups = ctypes.c_int32()
+        band = ctypes.c_int32()
+        check(lib().pa_solver_last_schedule(self._h, ctypes.byref(groups), ctypes.byref(band)))
+        return groups.value, band.value
+
     def last_kernel_ms(self):
         ms = ctypes.c_double()
         launches = ctypes.c_int32()

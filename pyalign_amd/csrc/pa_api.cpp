@@ -139,6 +139,8 @@ struct Plan {
     int64_t tb_words = 0;
     int64_t dvals = 0;
     double cells = 0.0;
+    int64_t band_slots = 0;            // strips of band-mode problems
+    std::vector<int32_t> band_probs;   // their indices in probs
 };
 
 // The last call's schedule, reused (with its device copy) while the next
@@ -167,11 +169,16 @@ struct pa_solver {
     bool has_device = false;
     DevBuf b_border_s, b_border_t, b_probs, b_groups, b_queue, b_cs, b_ct, b_table, b_dense, b_rowbuf;
     DevBuf b_scores, b_seeds, b_tb, b_path, b_path_off, b_path_len, b_dvals;
+    DevBuf b_progress, b_partials, b_band_err, b_band_probs;
+    uint32_t *band_err_host = nullptr;  // pinned: the last band-mode launch's failure flag
+    bool band_check = false;            // a band-mode flag is on its way to band_err_host
     double last_ms = 0.0;
     int last_launches = 0;
     double last_cells = 0.0;
+    int last_groups = 0, last_band = 0;
 
     ~pa_solver() {
+        if (band_err_host) (void)hipHostFree(band_err_host);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -263,6 +270,15 @@ void make_borders(pa_solver &s) {
     s.border_s[0] = s.border_t[0];
 }
 
+// After a synchronisation: a band-mode wait that gave up leaves wrong
+// results behind, so the call fails.
+void check_band(pa_solver &s) {
+    if (!s.band_check) return;
+    s.band_check = false;
+    if (*s.band_err_host)
+        fail(PA_ERUNTIME, "band-mode strip pipeline stalled (a strip waited too long for the one above it)");
+}
+
 void ensure_device(pa_solver &s) {
     if (s.has_device) return;
     int count = 0;
@@ -276,6 +292,8 @@ void ensure_device(pa_solver &s) {
     HIPCHK(hipStreamCreate(&s.stream));
     HIPCHK(hipEventCreate(&s.ev0));
     HIPCHK(hipEventCreate(&s.ev1));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&s.band_err_host), 4, hipHostMallocDefault));
+    *s.band_err_host = 0;
     s.b_border_s.ensure(s.border_s.size() * 4);
     s.b_border_t.ensure(s.border_t.size() * 4);
     HIPCHK(hipMemcpy(s.b_border_s.p, s.border_s.data(), s.border_s.size() * 4, hipMemcpyHostToDevice));
@@ -302,8 +320,8 @@ int ilog2(int x) {
 // layout take the one with the least serial time.
 constexpr double STEP_OVERHEAD = 4.0;
 
-void choose_layout(int64_t ls, int64_t lt, const std::vector<int> &Rs, double cap, int &L_out, int &R_out,
-                   double *cost_out = nullptr) {
+bool choose_layout(int64_t ls, int64_t lt, const std::vector<int> &Rs, double cap, int &L_out, int &R_out,
+                   double *cost_out = nullptr, double *serial_out = nullptr) {
     double best_cost = 1e300, best_serial = 1e300;
     int bL = 64, bR = Rs.back();
     double fb_serial = 1e300, fb_cost = 1e300;
@@ -329,14 +347,31 @@ void choose_layout(int64_t ls, int64_t lt, const std::vector<int> &Rs, double ca
             }
         }
     }
-    if (best_cost == 1e300) {
+    const bool fit = best_cost != 1e300;
+    if (!fit) {
         bL = fL;
         bR = fR;
         best_cost = fb_cost;
+        best_serial = fb_serial;
     }
     L_out = bL;
     R_out = bR;
     if (cost_out) *cost_out = best_cost;
+    if (serial_out) *serial_out = best_serial;
+    return fit;
+}
+
+// Band mode for a problem no layout brings under the cap: strips of 64 * 8
+// rows run concurrently, each starting about 64 + BAND_PUB steps after the
+// strip above, so the serial time is (len_t + 63 + (strips - 1) * (64 +
+// BAND_PUB)) steps instead of strips * (len_t + 63).
+constexpr int BAND_L = 64, BAND_R = 8;
+
+double band_serial(int64_t ls, int64_t lt, int *nst_out) {
+    const int64_t rps = int64_t(BAND_L) * BAND_R;
+    const int64_t nst = (ls + rps - 1) / rps;
+    *nst_out = int(nst);
+    return double(lt + BAND_L - 1 + (nst - 1) * (BAND_L + BAND_PUB)) * (STEP_OVERHEAD + BAND_R);
 }
 
 struct Job {
@@ -368,16 +403,33 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         int64_t k;
         int L, R, nst;
         int64_t lt;
+        double band;  // > 0: band mode, its serial time
     };
+    const bool band_ok = job.tr != TR_ALL && supported_R(job.form, job.tr, false, BAND_R);
     std::vector<Item> items(static_cast<size_t>(n));
+    std::vector<Item> bands;
     for (int64_t k = 0; k < n; k++) {
         int L = 64, R = Rs.back();
-        if (ls[k] > 0 && lt[k] > 0) choose_layout(ls[k], lt[k], Rs, cap, L, R);
+        double serial = 0.0;
+        bool fit = true;
+        if (ls[k] > 0 && lt[k] > 0) fit = choose_layout(ls[k], lt[k], Rs, cap, L, R, nullptr, &serial);
         else { L = 1; R = Rs.front(); }
         const int64_t rps = int64_t(L) * R;
         const int nst = ls[k] > 0 && lt[k] > 0 ? int((ls[k] + rps - 1) / rps) : 0;
-        items[size_t(k)] = Item{k, L, R, nst, lt[k]};
+        if (!fit && band_ok) {
+            int nb = 0;
+            const double sb = band_serial(ls[k], lt[k], &nb);
+            if (nb >= 2 && sb < serial) {
+                bands.push_back(Item{k, BAND_L, BAND_R, nb, lt[k], sb});
+                items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0};
+                continue;
+            }
+        }
+        items[size_t(k)] = Item{k, L, R, nst, lt[k], 0.0};
     }
+    if (!bands.empty())
+        items.erase(std::remove_if(items.begin(), items.end(), [](const Item &it) { return it.L == 0; }),
+                    items.end());
     // group identical layouts with similar shape: sort by R, L, strips, len_t
     std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
         if (a.R != b.R) return a.R < b.R;
@@ -400,12 +452,14 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         g.first = int32_t(plan.probs.size());
         g.L = L;
         g.lgL = ilog2(L);
+        g.strip = -1;
         int cnt = 0;
         while (p < items.size() && cnt < per && items[p].L == L && items[p].R == R) {
             Prob pr{};
             pr.out = int32_t(items[p].k);
             pr.L = L;
             pr.R = R;
+            pr.band = -1;
             plan.probs.push_back(pr);
             g.max_lt = std::max<int32_t>(g.max_lt, int32_t(items[p].lt));
             g.max_nst = std::max<int32_t>(g.max_nst, items[p].nst);
@@ -416,6 +470,34 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         g.R = R;
         g.cost = int64_t(double(g.max_nst) * double(g.max_lt + L - 1) * (STEP_OVERHEAD + R));
         gs.push_back(GroupR{g, job.fast ? 0 : R});
+    }
+    // band-mode problems: one group per strip, strips in order, run by the
+    // general kernel (its launch goes first: it holds the longest problems)
+    std::stable_sort(bands.begin(), bands.end(), [](const Item &a, const Item &b) { return a.band > b.band; });
+    for (const Item &it : bands) {
+        Prob pr{};
+        pr.out = int32_t(it.k);
+        pr.L = BAND_L;
+        pr.R = BAND_R;
+        pr.band = int32_t(plan.band_slots);
+        plan.band_slots += it.nst;
+        plan.band_probs.push_back(int32_t(plan.probs.size()));
+        for (int st = 0; st < it.nst; st++) {
+            Group g{};
+            g.first = int32_t(plan.probs.size());
+            g.count = 1;
+            g.L = BAND_L;
+            g.lgL = ilog2(BAND_L);
+            g.max_lt = int32_t(it.lt);
+            g.max_nst = it.nst;
+            g.R = BAND_R;
+            g.strip = st;
+            g.cost = int64_t(it.band);
+            // with the other groups of strip height BAND_R, or (fast jobs) in a
+            // general-kernel launch of its own, marked by 1000 + R
+            gs.push_back(GroupR{g, job.fast ? 1000 + BAND_R : BAND_R});
+        }
+        plan.probs.push_back(pr);
     }
     // one launch per kernel (the fast kernel serves every R); inside a
     // launch the longest groups go first (LPT)
@@ -466,6 +548,7 @@ Plan plan_plain(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n)
         Group g{};
         g.first = int32_t(plan.probs.size());
         g.count = int32_t(idx.size());
+        g.strip = -1;
         g.L = 1;
         g.lgL = 0;
         g.max_lt = int32_t(mt);
@@ -475,6 +558,7 @@ Plan plan_plain(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n)
         for (int64_t i : idx) {
             Prob pr{};
             pr.out = int32_t(i);
+            pr.band = -1;
             pr.L = 1;
             pr.R = R;
             plan.probs.push_back(pr);
@@ -606,7 +690,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             }
             if (nst > 1) {
                 pr.rb_off = pl.rowbuf;
-                pl.rowbuf += pr.len_t;
+                pl.rowbuf += pr.band >= 0 ? (nst - 1) * int64_t(pr.len_t) : int64_t(pr.len_t);
             }
             if (keep_vals) {
                 pr.dv_off = pl.dvals;
@@ -616,6 +700,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         }
         upload(st, s.b_probs, pl.probs.data(), pl.probs.size());
         upload(st, s.b_groups, pl.groups.data(), pl.groups.size());
+        if (!pl.band_probs.empty()) upload(st, s.b_band_probs, pl.band_probs.data(), pl.band_probs.size());
         pc.key = key;
         pc.len_s.assign(in.len_s, in.len_s + n);
         pc.len_t.assign(in.len_t, in.len_t + n);
@@ -681,11 +766,26 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     a.tb = job.tr != TR_SCORE ? s.b_tb.as<uint32_t>() : nullptr;
     a.dvals = plan.dvals > 0 ? s.b_dvals.as<float>() : nullptr;
     a.val_layers = val_layers;
+    if (plan.band_slots > 0) {
+        s.b_progress.ensure(size_t(plan.band_slots) * 4);
+        s.b_partials.ensure(size_t(plan.band_slots) * sizeof(BandPartial));
+        s.b_band_err.ensure(4);
+        HIPCHK(launch_zero_u32(s.b_progress.as<uint32_t>(), plan.band_slots, st));
+        HIPCHK(launch_zero_u32(s.b_band_err.as<uint32_t>(), 1, st));
+        a.progress = s.b_progress.as<uint32_t>();
+        a.partials = s.b_partials.as<BandPartial>();
+        a.band_err = s.b_band_err.as<uint32_t>();
+    }
 
     HIPCHK(hipEventRecord(s.ev0, st));
     int launches = 0;
     for (size_t li = 0; li < plan.launches.size(); li++) {
-        const int R = plan.launches[li].first;
+        int R = plan.launches[li].first;
+        bool fastmax = job.fast;
+        if (R >= 1000) {  // band-mode strips of a fast job: the general kernel
+            R -= 1000;
+            fastmax = false;
+        }
         const int g0 = plan.launches[li].second.first, g1 = plan.launches[li].second.second;
         if (g1 <= g0) continue;
         LaunchSpec spec{};
@@ -695,7 +795,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         spec.form = job.form;
         spec.tr = job.tr;
         spec.R = R;
-        spec.fastmax = job.fast;
+        spec.fastmax = fastmax;
         spec.lds_bytes = job.form == F_TABLE ? size_t(in.K1) * in.K1 * 4 : 0;
         const std::vector<int> okey = {spec.loc, spec.dir, spec.gap, spec.form, spec.tr, spec.R,
                                        int(spec.fastmax), int(spec.lds_bytes)};
@@ -721,9 +821,17 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         }
         launches++;
     }
+    if (!plan.band_probs.empty()) {
+        HIPCHK(launch_band_combine(job.loc, job.dir, job.tr, s.b_probs.as<Prob>(), s.b_band_probs.as<int32_t>(),
+                                   int(plan.band_probs.size()), s.b_partials.as<BandPartial>(), scores, seeds, st));
+        HIPCHK(hipMemcpyAsync(s.band_err_host, s.b_band_err.p, 4, hipMemcpyDeviceToHost, st));
+        s.band_check = true;
+    }
     HIPCHK(hipEventRecord(s.ev1, st));
     s.last_launches = launches;
     s.last_cells = plan.cells;
+    s.last_groups = int(plan.groups.size());
+    s.last_band = int(plan.band_probs.size());
 
     if (tape) {
         tape->solver = &s;
@@ -799,6 +907,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     }
     if (n) HIPCHK(hipMemcpyAsync(r->scores.data(), scores, size_t(n) * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    check_band(s);
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
     s.last_ms = ms;
@@ -1192,6 +1301,7 @@ int pa_solver_sync(pa_solver *s) {
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
         s->last_ms = ms;
+        check_band(*s);
     });
 }
 
@@ -1201,6 +1311,14 @@ int pa_solver_last_kernel_ms(const pa_solver *s, double *ms, int32_t *launches, 
         if (ms) *ms = s->last_ms;
         if (launches) *launches = s->last_launches;
         if (cells) *cells = s->last_cells;
+    });
+}
+
+int pa_solver_last_schedule(const pa_solver *s, int32_t *groups, int32_t *band_problems) {
+    return guard([&] {
+        if (!s) fail(PA_EINVAL, "null solver");
+        if (groups) *groups = s->last_groups;
+        if (band_problems) *band_problems = s->last_band;
     });
 }
 

@@ -33,7 +33,7 @@ struct Prob {
     int32_t out;     // output slot (caller's problem index)
     int32_t L;       // lanes cooperating on this problem
     int32_t R;       // rows per lane
-    int32_t pad;
+    int32_t band;    // band mode: first strip slot (progress, partials); -1 otherwise
 };
 
 // One wavefront's work: 64/L problems (segment k <- probs[first + k]).
@@ -43,9 +43,27 @@ struct Group {
     int32_t max_lt;      // over members: step loop bound is max_lt + L - 1
     int32_t max_nst;     // over members: strips of L*R rows
     int32_t R;           // rows per lane (kernels built for several R dispatch on it)
-    int32_t pad;
+    int32_t strip;       // band mode: the one strip this group computes; -1: all strips
     int64_t cost;        // scheduling cost
 };
+
+// Band mode (long problems, few of them): each strip of L*R rows is its own
+// group, so the strips of one problem run concurrently on different
+// wavefronts.  Strip st reads the bottom row of strip st-1 from its row
+// buffer as soon as it is published: progress[band + st - 1] counts the
+// published columns (release/acquire at agent scope).  Groups are queued in
+// strip order, so a strip only ever waits for a strip taken earlier by a
+// running wavefront.  Each strip leaves its share of the seed search in
+// partials[band + st]; pa_band_combine_kernel folds them in strip order.
+struct BandPartial {
+    float v1;        // local: best; global: D(ls-1, lt-1); semiglobal: last-column best
+    float v2;        // semiglobal: last-row best
+    int32_t i1, j1;  // local: best cell; semiglobal: last-column row
+    int32_t j2;      // semiglobal: last-row column
+    int32_t pad;
+};
+constexpr int BAND_PUB = 32;                 // columns per progress publication
+constexpr uint32_t BAND_SPIN_LIMIT = 1u << 20;  // polls before a wait gives up (about a second)
 
 struct KArgs {
     const Prob *probs;
@@ -68,6 +86,9 @@ struct KArgs {
     uint32_t *tb;      // traceback words
     float *dvals;      // cell values (null: none kept): D, then P and Q when val_layers == 3
     int32_t val_layers;
+    uint32_t *progress;     // band mode: published columns per strip
+    BandPartial *partials;  // band mode: seed search per strip
+    uint32_t *band_err;     // band mode: set when a wait gave up
 };
 
 // R = 0 selects the kernel that serves every strip height in FAST_RS from
@@ -88,6 +109,9 @@ hipError_t launch_solve(const LaunchSpec &spec, const KArgs &args, hipStream_t s
 hipError_t launch_walk(int gap, const Prob *probs, int64_t n, const uint32_t *tb, const int32_t *seeds,
                        int32_t *path, const int64_t *path_off, int32_t *path_len, hipStream_t stream);
 hipError_t launch_zero_u32(uint32_t *p, int64_t n, hipStream_t stream);
+// Folds band-mode strips' partials into scores and seeds (one thread per problem).
+hipError_t launch_band_combine(int loc, int dir, int tr, const Prob *probs, const int32_t *band_probs, int n,
+                               const BandPartial *partials, float *scores, int32_t *seeds, hipStream_t stream);
 
 // Backward pass over one-optimal tracebacks (one thread per problem).
 struct BwArgs {

@@ -66,7 +66,27 @@ __device__ void write_empty(const KArgs &a, const Prob &pr) {
     }
 }
 
-template <int LOC, int DIR, int GAP, int FORM, int TR, int R, bool FAST, int SD>
+// Band mode: wait until strip `idx` has published `need` columns (lane 0
+// only).  Bounded: after BAND_SPIN_LIMIT polls the wait gives up and flags
+// the failure instead of spinning on.
+__device__ __forceinline__ void band_wait(const KArgs &a, int idx, uint32_t need, uint32_t &seen) {
+    uint32_t spins = 0;
+    while (seen < need) {
+        seen = __hip_atomic_load(a.progress + idx, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (seen >= need) break;
+        ++spins;
+        // give up after the limit, or at once when another wait already has
+        if (spins > BAND_SPIN_LIMIT ||
+            ((spins & 255u) == 0 && __hip_atomic_load(a.band_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(a.band_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            seen = 0xffffffffu;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+template <int LOC, int DIR, int GAP, int FORM, int TR, int R, bool FAST, int SD, bool BAND = false>
 __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, const float *__restrict__ tab,
                                             int lane) {
     constexpr float W = worst<DIR>();
@@ -85,6 +105,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         pr.off_s = pr.off_t = pr.tb_off = pr.rb_off = pr.dv_off = 0;
         pr.len_s = pr.len_t = 0;
         pr.si = pr.sj = pr.out = 0;
+        pr.band = -1;
     }
     const int ls = pr.len_s, lt = pr.len_t;
     const int rps = L * R;
@@ -105,9 +126,17 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
     const int l_last = last >= 0 ? (last % rps) / R : -1;
     const int r_last = last >= 0 ? last % R : -1;
 
-    for (int st = 0; st < g.max_nst; ++st) {
+    // band mode computes one strip (its rows' part of the seed search
+    // goes to partials); otherwise the segment sweeps every strip
+    const int st_lo = BAND ? g.strip : 0;
+    const int st_hi = BAND ? g.strip + 1 : g.max_nst;
+    for (int st = st_lo; st < st_hi; ++st) {
         const bool act = st < nst;
         const int row0 = st * rps + l * R;
+        // row buffers: one reused row, or one row per strip boundary in band mode
+        const int64_t rb_in = BAND ? pr.rb_off + int64_t(st - 1) * lt : pr.rb_off;
+        const int64_t rb_out = BAND ? pr.rb_off + int64_t(st) * lt : pr.rb_off;
+        [[maybe_unused]] uint32_t seen = 0;  // band mode: columns of strip st-1 known published
 
         int rowsym[R];
         bool rv[R];
@@ -170,7 +199,8 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                     if (st == 0) {
                         nx_D = a.border_t[j + 1];  // D(-1, j)
                     } else {
-                        const float2 v = a.rowbuf[pr.rb_off + j];
+                        if constexpr (BAND) band_wait(a, pr.band + st - 1, uint32_t(j + 1), seen);
+                        const float2 v = a.rowbuf[rb_in + j];
                         nx_D = v.x;
                         nx_P = v.y;
                     }
@@ -338,7 +368,14 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 }
                 botD = Dup;
                 botP = Pup;
-                if (l == L - 1 && st + 1 < nst) a.rowbuf[pr.rb_off + j] = make_float2(botD, botP);
+                if (l == L - 1 && st + 1 < nst) {
+                    a.rowbuf[rb_out + j] = make_float2(botD, botP);
+                    if constexpr (BAND) {
+                        if (((j + 1) % BAND_PUB) == 0 || j == lt - 1)
+                            __hip_atomic_store(a.progress + pr.band + st, uint32_t(j + 1), __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
 
                 if constexpr (TR != TR_SCORE) {
                     uint32_t *dst = a.tb + pr.tb_off + (int64_t(st) * nsteps_p + s) * int64_t(L) * WPC + l * WPC;
@@ -417,6 +454,19 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         // improvement
         if (better<DIR>(cbest, rbest)) { score = cbest; si = ci; sj = lt - 1; }
         else { score = rbest; si = ls - 1; sj = rj; }
+    }
+    if constexpr (BAND) {
+        if (has && l == 0 && g.strip < nst) {
+            BandPartial bp;
+            bp.v1 = LOC == LOC_LOCAL ? best : (LOC == LOC_GLOBAL ? score : cbest);
+            bp.i1 = LOC == LOC_LOCAL ? bi : ci;
+            bp.j1 = bj;
+            bp.v2 = rbest;
+            bp.j2 = rj;
+            bp.pad = 0;
+            a.partials[pr.band + g.strip] = bp;
+        }
+        return;
     }
     if (has && l == 0) {
         if (ls == 0 || lt == 0) {
@@ -584,7 +634,14 @@ __global__ __launch_bounds__(256) void pa_solve_kernel(KArgs a) {
         if (lane == 0) t = atomicAdd(a.queue, 1u);
         t = uint32_t(__shfl(int(t), 0, WAVE));
         if (t >= uint32_t(a.n_groups)) break;
-        solve_group<LOC, DIR, GAP, FORM, TR, R, FAST, SD>(a, a.groups[t], lds_table, lane);
+        const Group &g = a.groups[t];
+        if constexpr (!FAST && TR != TR_ALL) {
+            if (g.strip >= 0) {
+                solve_group<LOC, DIR, GAP, FORM, TR, R, FAST, SD, true>(a, g, lds_table, lane);
+                continue;
+            }
+        }
+        solve_group<LOC, DIR, GAP, FORM, TR, R, FAST, SD>(a, g, lds_table, lane);
     }
 }
 

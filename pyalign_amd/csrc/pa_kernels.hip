@@ -1,6 +1,6 @@
 // pa_kernels.hip -- traceback walk, small helper kernels and the top-level
 // dispatch into the per-locality instantiation units (pa_k_*.hip).
-#include "pa_internal.h"
+#include "pa_kernels.h"
 
 namespace pa {
 
@@ -397,6 +397,89 @@ hipError_t launch_backward(int gap, int form, int dim, const BwArgs &args, hipSt
         return hipErrorInvalidValue;
     }
 #undef PA_BW
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Band mode: fold each problem's strip partials in strip order with the
+// single-segment rules (local: larger i then larger j on ties; global: the
+// last row's strip; semiglobal: last row first, then the last column in
+// ascending i with strict improvement, solver.h:2042-2057, 2179-2187,
+// 2273-2293).
+// ---------------------------------------------------------------------------
+template <int LOC, int DIR>
+__global__ __launch_bounds__(256) void pa_band_combine_kernel(const Prob *probs, const int32_t *band_probs, int n,
+                                                              const BandPartial *parts, float *scores,
+                                                              int32_t *seeds, int score_only) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const Prob pr = probs[band_probs[t]];
+    const int ls = pr.len_s, lt = pr.len_t;
+    const int rps = pr.L * pr.R;
+    const int nst = (ls + rps - 1) / rps;
+    const BandPartial *p = parts + pr.band;
+    constexpr float W = worst<DIR>();
+    float score = W;
+    int si = NO_SEED, sj = 0;
+    if constexpr (LOC == LOC_LOCAL) {
+        float best = W;
+        int bi = -1, bj = -1;
+        for (int st = 0; st < nst; ++st) {
+            if (better<DIR>(p[st].v1, best) || (p[st].v1 == best && p[st].i1 > bi)) {
+                best = p[st].v1;
+                bi = p[st].i1;
+                bj = p[st].j1;
+            }
+        }
+        score = best;
+        si = score_only ? 0 : bi;
+        sj = score_only ? 0 : bj;
+    } else if constexpr (LOC == LOC_GLOBAL) {
+        score = p[nst - 1].v1;
+        si = ls - 1;
+        sj = lt - 1;
+    } else {
+        float cb = W;
+        int ci = -1;
+        for (int st = 0; st < nst; ++st) {
+            if (better<DIR>(p[st].v1, cb) || (p[st].v1 == cb && p[st].i1 >= 0 && (ci < 0 || p[st].i1 < ci))) {
+                cb = p[st].v1;
+                ci = p[st].i1;
+            }
+        }
+        const float rb = p[nst - 1].v2;
+        if (better<DIR>(cb, rb)) {
+            score = cb;
+            si = ci;
+            sj = lt - 1;
+        } else {
+            score = rb;
+            si = ls - 1;
+            sj = p[nst - 1].j2;
+        }
+    }
+    scores[pr.out] = score;
+    if (seeds) {
+        seeds[2 * pr.out] = si;
+        seeds[2 * pr.out + 1] = sj;
+    }
+}
+
+hipError_t launch_band_combine(int loc, int dir, int tr, const Prob *probs, const int32_t *band_probs, int n,
+                               const BandPartial *partials, float *scores, int32_t *seeds, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid(unsigned((n + 255) / 256)), block(256);
+    const int so = tr == TR_SCORE ? 1 : 0;
+#define PA_BC(LC, DR) \
+    hipLaunchKernelGGL((pa_band_combine_kernel<LC, DR>), grid, block, 0, st, probs, band_probs, n, partials, scores, \
+                       seeds, so)
+    const bool mx = dir == DIR_MAX;
+    switch (loc) {
+        case LOC_LOCAL: if (mx) PA_BC(LOC_LOCAL, DIR_MAX); else PA_BC(LOC_LOCAL, DIR_MIN); break;
+        case LOC_GLOBAL: if (mx) PA_BC(LOC_GLOBAL, DIR_MAX); else PA_BC(LOC_GLOBAL, DIR_MIN); break;
+        default: if (mx) PA_BC(LOC_SEMI, DIR_MAX); else PA_BC(LOC_SEMI, DIR_MIN); break;
+    }
+#undef PA_BC
     return hipGetLastError();
 }
 
