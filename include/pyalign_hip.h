@@ -172,6 +172,10 @@ int pa_solver_last_kernel_ms(const pa_solver *solver, double *ms, int32_t *launc
  * mode (a long problem's strips computed concurrently by several
  * wavefronts).  No reference counterpart (diagnostics). */
 int pa_solver_last_schedule(const pa_solver *solver, int32_t *groups, int32_t *band_problems);
+/* Repeated launch sequences (same shapes and buffers) are captured as a
+ * hipGraph and replayed; enable = 0 launches every kernel eagerly (also the
+ * default when PYALIGN_AMD_NO_GRAPHS is set).  No reference counterpart. */
+int pa_solver_set_graphs(pa_solver *solver, int32_t enable);
 
 /*
  * Differentiation (no reference counterpart: the reference has no

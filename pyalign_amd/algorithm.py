@@ -521,6 +521,10 @@ class Solver:
             one = _ragged_one(form, len_s, len_t, payload)
         return self._wrap(h, len_s, len_t, kind, one)
 
+    def set_graphs(self, enable):
+        """Replay repeated launch sequences from a captured hipGraph (default on)."""
+        check(lib().pa_solver_set_graphs(self._h, int(bool(enable))))
+
     def last_schedule(self):
         """(work groups, problems in band mode) of the last solve."""
         groups = ctypes.c_int32()

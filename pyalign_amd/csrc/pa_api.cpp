@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -172,12 +173,17 @@ struct pa_solver {
     DevBuf b_progress, b_partials, b_band_err, b_band_probs;
     uint32_t *band_err_host = nullptr;  // pinned: the last band-mode launch's failure flag
     bool band_check = false;            // a band-mode flag is on its way to band_err_host
+    // launch sequences replayed from a hipGraph when they repeat
+    bool use_graphs = true;
+    std::string prev_key, graph_key;
+    hipGraphExec_t graph_exec = nullptr;
     double last_ms = 0.0;
     int last_launches = 0;
     double last_cells = 0.0;
     int last_groups = 0, last_band = 0;
 
     ~pa_solver() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (band_err_host) (void)hipHostFree(band_err_host);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -714,7 +720,6 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
 
     // ---- device buffers ----------------------------------------------------
     s.b_queue.ensure(std::max<size_t>(plan.launches.size(), 1) * 4);
-    HIPCHK(launch_zero_u32(s.b_queue.as<uint32_t>(), int64_t(plan.launches.size()), st));
     const void *codes_s = in.codes_s, *codes_t = in.codes_t;
     const float *dense = in.dense;
     if (!in.on_device) {
@@ -743,7 +748,8 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     }
     if (plan.dvals > 0) s.b_dvals.ensure(size_t(plan.dvals) * 4);
 
-    KArgs a{};
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));  // padding included: the bytes key the graph cache
     a.probs = s.b_probs.as<Prob>();
     a.groups = s.b_groups.as<Group>();
     a.codes_s = codes_s;
@@ -770,15 +776,14 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         s.b_progress.ensure(size_t(plan.band_slots) * 4);
         s.b_partials.ensure(size_t(plan.band_slots) * sizeof(BandPartial));
         s.b_band_err.ensure(4);
-        HIPCHK(launch_zero_u32(s.b_progress.as<uint32_t>(), plan.band_slots, st));
-        HIPCHK(launch_zero_u32(s.b_band_err.as<uint32_t>(), 1, st));
         a.progress = s.b_progress.as<uint32_t>();
         a.partials = s.b_partials.as<BandPartial>();
         a.band_err = s.b_band_err.as<uint32_t>();
     }
 
-    HIPCHK(hipEventRecord(s.ev0, st));
-    int launches = 0;
+    // ---- launches: specs first (occupancy queries stay outside any capture)
+    std::vector<LaunchSpec> specs;
+    std::vector<KArgs> largs;
     for (size_t li = 0; li < plan.launches.size(); li++) {
         int R = plan.launches[li].first;
         bool fastmax = job.fast;
@@ -788,7 +793,8 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         }
         const int g0 = plan.launches[li].second.first, g1 = plan.launches[li].second.second;
         if (g1 <= g0) continue;
-        LaunchSpec spec{};
+        LaunchSpec spec;
+        std::memset(&spec, 0, sizeof(spec));
         spec.loc = job.loc;
         spec.dir = job.dir;
         spec.gap = job.gap;
@@ -812,18 +818,67 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         la.groups = a.groups + g0;
         la.n_groups = g1 - g0;
         la.queue = s.b_queue.as<uint32_t>() + li;
-        const hipError_t e = launch_solve(spec, la, st);
-        if (e != hipSuccess) {
-            std::ostringstream m;
-            m << "kernel launch failed (loc " << spec.loc << " dir " << spec.dir << " gap " << spec.gap << " form "
-              << spec.form << " tr " << spec.tr << " R " << spec.R << "): " << hipGetErrorString(e);
-            fail(PA_EHIP, m.str());
-        }
-        launches++;
+        specs.push_back(spec);
+        largs.push_back(la);
     }
-    if (!plan.band_probs.empty()) {
-        HIPCHK(launch_band_combine(job.loc, job.dir, job.tr, s.b_probs.as<Prob>(), s.b_band_probs.as<int32_t>(),
-                                   int(plan.band_probs.size()), s.b_partials.as<BandPartial>(), scores, seeds, st));
+    const int n_band = int(plan.band_probs.size());
+    // the whole sequence: queue tickets and band counters cleared by kernels
+    // (no memset nodes), the DP launches, the band fold
+    auto enqueue = [&]() -> hipError_t {
+        hipError_t e = launch_zero_u32(s.b_queue.as<uint32_t>(), int64_t(plan.launches.size()), st);
+        if (e == hipSuccess && plan.band_slots > 0) e = launch_zero_u32(s.b_progress.as<uint32_t>(), plan.band_slots, st);
+        if (e == hipSuccess && plan.band_slots > 0) e = launch_zero_u32(s.b_band_err.as<uint32_t>(), 1, st);
+        for (size_t q = 0; q < specs.size() && e == hipSuccess; q++) e = launch_solve(specs[q], largs[q], st);
+        if (e == hipSuccess && n_band > 0)
+            e = launch_band_combine(job.loc, job.dir, job.tr, s.b_probs.as<Prob>(), s.b_band_probs.as<int32_t>(),
+                                    n_band, s.b_partials.as<BandPartial>(), scores, seeds, st);
+        return e;
+    };
+    // A launch sequence repeated with identical arguments (same shapes, same
+    // buffers) is captured once as a hipGraph and replayed; the kernels read
+    // the plan and inputs uploaded just before, so a replay sees new data.
+    std::string gkey(reinterpret_cast<const char *>(specs.data()), specs.size() * sizeof(LaunchSpec));
+    gkey.append(reinterpret_cast<const char *>(largs.data()), largs.size() * sizeof(KArgs));
+    const int64_t extra[6] = {int64_t(plan.launches.size()), plan.band_slots, n_band,
+                              int64_t(reinterpret_cast<intptr_t>(scores)), int64_t(reinterpret_cast<intptr_t>(seeds)),
+                              int64_t(job.loc * 100 + job.dir * 10 + job.tr)};
+    gkey.append(reinterpret_cast<const char *>(extra), sizeof(extra));
+
+    HIPCHK(hipEventRecord(s.ev0, st));
+    hipError_t le = hipSuccess;
+    if (s.use_graphs && s.graph_exec && gkey == s.graph_key) {
+        le = hipGraphLaunch(s.graph_exec, st);
+    } else if (s.use_graphs && gkey == s.prev_key) {
+        if (s.graph_exec) {
+            (void)hipGraphExecDestroy(s.graph_exec);
+            s.graph_exec = nullptr;
+            s.graph_key.clear();
+        }
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        le = enqueue();
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(st, &graph);
+        if (le == hipSuccess) le = ce;
+        if (le == hipSuccess) le = hipGraphInstantiate(&s.graph_exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (le == hipSuccess) {
+            s.graph_key = gkey;
+            le = hipGraphLaunch(s.graph_exec, st);
+        } else {
+            s.graph_exec = nullptr;
+        }
+    } else {
+        le = enqueue();
+    }
+    s.prev_key = gkey;
+    if (le != hipSuccess) {
+        std::ostringstream m;
+        m << "kernel launch failed (loc " << job.loc << " dir " << job.dir << " gap " << job.gap << " form "
+          << job.form << " tr " << job.tr << "): " << hipGetErrorString(le);
+        fail(PA_EHIP, m.str());
+    }
+    const int launches = int(specs.size());
+    if (n_band > 0) {
         HIPCHK(hipMemcpyAsync(s.band_err_host, s.b_band_err.p, 4, hipMemcpyDeviceToHost, st));
         s.band_check = true;
     }
@@ -1023,6 +1078,7 @@ int pa_create_solver(int64_t max_len_s, int64_t max_len_t, const pa_options *opt
         if (max_len_s < 0 || max_len_t < 0) fail(PA_EINVAL, "negative maximum length");
         auto s = std::make_unique<pa_solver>();
         s->opt = *opts;
+        s->use_graphs = std::getenv("PYALIGN_AMD_NO_GRAPHS") == nullptr;
         s->lim_s = max_len_s;
         s->lim_t = max_len_t;
         make_borders(*s);
@@ -1311,6 +1367,14 @@ int pa_solver_last_kernel_ms(const pa_solver *s, double *ms, int32_t *launches, 
         if (ms) *ms = s->last_ms;
         if (launches) *launches = s->last_launches;
         if (cells) *cells = s->last_cells;
+    });
+}
+
+int pa_solver_set_graphs(pa_solver *s, int32_t enable) {
+    return guard([&] {
+        if (!s) fail(PA_EINVAL, "null solver");
+        s->use_graphs = enable != 0;
+        s->prev_key.clear();
     });
 }
 
