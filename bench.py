@@ -28,8 +28,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS batched Gotoh local align, fwd & fwd+bwd, vs plain GPU path; CPU ref"
-OPS_PER_CELL = 11          # 5 add + 4 max + local zero clamp + running best (Gotoh, local score)
-VALU_PEAK_TOPS = 157.3 / 4  # MI355X fp32 vector peak counts packed FMA (x2 x2); one lane-op/clk here
+OPS_PER_CELL = 11  # 5 add + 4 max + local zero clamp + running best (Gotoh, local score)
+# MI355X VALU ceiling in lane-operations: the published fp32 vector peak,
+# 157.3 TFLOP/s, counts an FMA as two operations; a SIMD issues at most two
+# wave64 VALU instructions per four cycles (SQ_ACTIVE_INST_VALU2 counts the
+# dual issues), i.e. 157.3 / 2 = 78.65 T single-operation lane instructions/s
+VALU_PEAK_TOPS = 157.3 / 2
 
 
 def log(msg):
@@ -315,7 +319,8 @@ def main():
                    "pairs": n, "cells": cells, "forward": "score", "launches_per_step": launches},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 3),
                      "unit": "Tops/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": traffic(),
-                     "ops_per_cell": OPS_PER_CELL, "kernel_ms_per_step": round(kms, 3)},
+                     "ops_per_cell": OPS_PER_CELL, "kernel_ms_per_step": round(kms, 3),
+                     "peak_basis": "fp32 157.3 TFLOP/s / 2: one add or max per lane op (an FMA counts 2)"},
         "plain_gpu": plain,
         "cpu_baseline": cpu,
         "fwd_bwd": fwd_bwd,

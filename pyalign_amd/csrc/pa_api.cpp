@@ -404,6 +404,11 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         total += cost / 64.0;
     }
     const double cap = std::max(256.0, total / (double(simds) * 4.0));
+    // band mode only for a problem that would otherwise be the critical path
+    // (serial time above the balanced per-SIMD total); its strips run in the
+    // general kernel, several times slower per cell than the fast one
+    const double crit = 4.0 * cap;
+    const double band_penalty = job.fast ? 3.0 : 1.0;
 
     struct Item {
         int64_t k;
@@ -422,9 +427,9 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         else { L = 1; R = Rs.front(); }
         const int64_t rps = int64_t(L) * R;
         const int nst = ls[k] > 0 && lt[k] > 0 ? int((ls[k] + rps - 1) / rps) : 0;
-        if (!fit && band_ok) {
+        if (!fit && band_ok && serial > crit) {
             int nb = 0;
-            const double sb = band_serial(ls[k], lt[k], &nb);
+            const double sb = band_serial(ls[k], lt[k], &nb) * band_penalty;
             if (nb >= 2 && sb < serial) {
                 bands.push_back(Item{k, BAND_L, BAND_R, nb, lt[k], sb});
                 items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0};

@@ -77,7 +77,10 @@ def test_band_scores_and_paths(loc, gap, direction, form):
     for codomain, kind, goal in ((pa.Score, "score", O.GOAL_SCORE), (pa.Solution, "solution", O.GOAL_ONE)):
         solver = make_solver(loc, gap, direction, codomain, lim_s, lim_t)
         res = solver.solve_ragged(kind, form, ls, lt, **payload)
-        assert solver.last_schedule()[1] == 3, solver.last_schedule()
+        # local maximisation over a table, score only, runs the fast kernel,
+        # whose tall strips (32 rows a lane) beat band mode at these shapes
+        fast = loc == "local" and direction == "maximize" and form == "indexed" and kind == "score"
+        assert solver.last_schedule()[1] == (0 if fast else 3), solver.last_schedule()
         for k in range(3):
             S = dense_of(form, k, cs, ct, os_, ot, ls, lt, table, eq, ne)
             ref = O.OracleResult(O.make_config(locality=LOCS[loc][1], direction=odir, gap=gap, goal=goal,
