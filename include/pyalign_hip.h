@@ -168,14 +168,23 @@ int pa_solver_sync(pa_solver *solver);
  * stream): total milliseconds and launch count. */
 int pa_solver_last_kernel_ms(const pa_solver *solver, double *ms, int32_t *launches,
                              double *cells);
-/* Schedule of the last solve: work groups queued and problems run in band
+/* Schedule of the last solve: work groups queued, problems run in band
  * mode (a long problem's strips computed concurrently by several
- * wavefronts).  No reference counterpart (diagnostics). */
-int pa_solver_last_schedule(const pa_solver *solver, int32_t *groups, int32_t *band_problems);
+ * wavefronts) and, of those, problems whose forward was checkpointed (see
+ * pa_solver_set_checkpoint).  No reference counterpart (diagnostics). */
+int pa_solver_last_schedule(const pa_solver *solver, int32_t *groups, int32_t *band_problems,
+                            int32_t *checkpointed);
 /* Repeated launch sequences (same shapes and buffers) are captured as a
  * hipGraph and replayed; enable = 0 launches every kernel eagerly (also the
  * default when PYALIGN_AMD_NO_GRAPHS is set).  No reference counterpart. */
 int pa_solver_set_graphs(pa_solver *solver, int32_t enable);
+/* Checkpointed differentiation: in pa_forward_ragged, a global problem on a
+ * device-resident dense or series payload whose longer side exceeds min_len
+ * (default 2048) keeps only the bottom row of each strip of 512 rows, not
+ * its traceback; pa_backward recomputes one strip's traceback at a time,
+ * last strip first, and continues the walk through it.  Negative: never.
+ * No reference counterpart. */
+int pa_solver_set_checkpoint(pa_solver *solver, int64_t min_len);
 
 /*
  * Differentiation (no reference counterpart: the reference has no

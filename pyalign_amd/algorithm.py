@@ -521,16 +521,22 @@ class Solver:
             one = _ragged_one(form, len_s, len_t, payload)
         return self._wrap(h, len_s, len_t, kind, one)
 
+    def set_checkpoint(self, min_len):
+        """Differentiation: recompute global pairs longer than min_len strip by
+        strip in the backward instead of keeping their traceback (-1: never)."""
+        check(lib().pa_solver_set_checkpoint(self._h, int(min_len)))
+
     def set_graphs(self, enable):
         """Replay repeated launch sequences from a captured hipGraph (default on)."""
         check(lib().pa_solver_set_graphs(self._h, int(bool(enable))))
 
     def last_schedule(self):
-        """(work groups, problems in band mode) of the last solve."""
+        """(work groups, problems in band mode, checkpointed problems) of the last solve."""
         groups = ctypes.c_int32()
         band = ctypes.c_int32()
-        check(lib().pa_solver_last_schedule(self._h, ctypes.byref(groups), ctypes.byref(band)))
-        return groups.value, band.value
+        ckpt = ctypes.c_int32()
+        check(lib().pa_solver_last_schedule(self._h, ctypes.byref(groups), ctypes.byref(band), ctypes.byref(ckpt)))
+        return groups.value, band.value, ckpt.value
 
     def last_kernel_ms(self):
         ms = ctypes.c_double()

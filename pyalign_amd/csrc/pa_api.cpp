@@ -175,12 +175,16 @@ struct pa_solver {
     bool band_check = false;            // a band-mode flag is on its way to band_err_host
     // launch sequences replayed from a hipGraph when they repeat
     bool use_graphs = true;
+    // a tape's global pairs longer than this (max of the two lengths) keep
+    // only strip boundary rows and are recomputed strip by strip in the
+    // backward; negative: never
+    int64_t ckpt_min = 2048;
     std::string prev_key, graph_key;
     hipGraphExec_t graph_exec = nullptr;
     double last_ms = 0.0;
     int last_launches = 0;
     double last_cells = 0.0;
-    int last_groups = 0, last_band = 0;
+    int last_groups = 0, last_band = 0, last_ckpt = 0;
 
     ~pa_solver() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -242,6 +246,14 @@ struct pa_tape {
     DevBuf probs, seeds, tb;
     const float *dense = nullptr;
     const float *xs = nullptr, *xt = nullptr;
+    // checkpointed problems: their strip boundary rows, the forward's kernel
+    // arguments (to recompute a strip) and their descriptors
+    DevBuf rowbuf;
+    pa::KArgs kargs{};
+    std::vector<pa::Prob> ckpt;
+    int64_t band_slots = 0;
+    // backward scratch
+    DevBuf s_probs, s_groups, s_queue, s_tb, s_progress, s_partials, s_err, s_state, s_strip;
 };
 
 namespace {
@@ -385,7 +397,11 @@ struct Job {
     bool fast;
 };
 
-Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, int simds) {
+// ckpt[k] != 0: problem k runs in band mode whatever its cost and keeps no
+// traceback (a checkpointed forward: the strip boundary rows are kept and
+// the backward recomputes one strip at a time).
+Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, int simds,
+               const uint8_t *ckpt = nullptr) {
     std::vector<int> Rs;
     if (job.fast) {
         Rs.assign(std::begin(FAST_RS), std::end(FAST_RS));
@@ -415,6 +431,7 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         int L, R, nst;
         int64_t lt;
         double band;  // > 0: band mode, its serial time
+        bool ckpt;    // band mode for a checkpointed forward
     };
     const bool band_ok = job.tr != TR_ALL && supported_R(job.form, job.tr, false, BAND_R);
     std::vector<Item> items(static_cast<size_t>(n));
@@ -427,16 +444,23 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         else { L = 1; R = Rs.front(); }
         const int64_t rps = int64_t(L) * R;
         const int nst = ls[k] > 0 && lt[k] > 0 ? int((ls[k] + rps - 1) / rps) : 0;
+        if (ckpt && ckpt[k]) {
+            int nb = 0;
+            const double sb = band_serial(ls[k], lt[k], &nb);
+            bands.push_back(Item{k, BAND_L, BAND_R, nb, lt[k], sb, true});
+            items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0, false};
+            continue;
+        }
         if (!fit && band_ok && serial > crit) {
             int nb = 0;
             const double sb = band_serial(ls[k], lt[k], &nb) * band_penalty;
             if (nb >= 2 && sb < serial) {
-                bands.push_back(Item{k, BAND_L, BAND_R, nb, lt[k], sb});
-                items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0};
+                bands.push_back(Item{k, BAND_L, BAND_R, nb, lt[k], sb, false});
+                items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0, false};
                 continue;
             }
         }
-        items[size_t(k)] = Item{k, L, R, nst, lt[k], 0.0};
+        items[size_t(k)] = Item{k, L, R, nst, lt[k], 0.0, false};
     }
     if (!bands.empty())
         items.erase(std::remove_if(items.begin(), items.end(), [](const Item &it) { return it.L == 0; }),
@@ -491,6 +515,7 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         pr.L = BAND_L;
         pr.R = BAND_R;
         pr.band = int32_t(plan.band_slots);
+        if (it.ckpt) pr.tb_off = NO_TB_OFF;
         plan.band_slots += it.nst;
         plan.band_probs.push_back(int32_t(plan.probs.size()));
         for (int st = 0; st < it.nst; st++) {
@@ -660,9 +685,25 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     const int bits = tb_bits(job.gap, job.tr);
     const int64_t *o_s = in.form == PA_FORM_DENSE ? in.dense_off : in.off_s;
     const int64_t *o_t = in.form == PA_FORM_DENSE ? nullptr : in.off_t;
+    // checkpointed forward (a tape of global problems on a device payload):
+    // pairs longer than the solver's threshold keep only their strip rows
+    std::vector<uint8_t> ckpt;
+    int64_t n_ckpt = 0;
+    if (tape && job.loc == LOC_GLOBAL && (job.form == F_DENSE || job.form == F_SERIES) && !in.matrices &&
+        s.ckpt_min >= 0 && in.schedule == PA_SCHED_AUTO) {
+        ckpt.assign(size_t(n), 0);
+        const int64_t rps = int64_t(BAND_L) * BAND_R;
+        for (int64_t k = 0; k < n; k++) {
+            const int64_t m = std::max(in.len_s[k], in.len_t[k]);
+            if (m > s.ckpt_min && in.len_t[k] > 0 && in.len_s[k] > rps) {
+                ckpt[size_t(k)] = 1;
+                n_ckpt++;
+            }
+        }
+    }
     const std::vector<int64_t> key = {n, job.loc, job.dir, job.gap, job.form, job.tr, int64_t(job.fast),
                                       in.schedule, in.dense_si_fixed, in.dense_sj_fixed, s.cus,
-                                      int64_t(in.matrices)};
+                                      int64_t(in.matrices), n_ckpt ? s.ckpt_min : -1};
     PlanCache &pc = s.last_plan;
     auto same = [&]() {
         if (!pc.valid || pc.key != key) return false;
@@ -677,7 +718,8 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     if (!reuse) {
         pc.valid = false;
         pc.plan = in.schedule == PA_SCHED_PLAIN ? plan_plain(job, in.len_s, in.len_t, n)
-                                                : plan_auto(job, in.len_s, in.len_t, n, s.cus * 4);
+                                                : plan_auto(job, in.len_s, in.len_t, n, s.cus * 4,
+                                                            n_ckpt ? ckpt.data() : nullptr);
         Plan &pl = pc.plan;
         // offsets: payload, traceback, row buffer, D values
         for (Prob &pr : pl.probs) {
@@ -695,7 +737,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             }
             const int64_t rps = int64_t(pr.L) * pr.R;
             const int64_t nst = (pr.len_s > 0 && pr.len_t > 0) ? (pr.len_s + rps - 1) / rps : 0;
-            if (bits) {
+            if (bits && pr.tb_off != NO_TB_OFF) {
                 pr.tb_off = pl.tb_words;
                 pl.tb_words += nst * (pr.len_t + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
             }
@@ -892,6 +934,8 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     s.last_cells = plan.cells;
     s.last_groups = int(plan.groups.size());
     s.last_band = int(plan.band_probs.size());
+    s.last_ckpt = int(std::count_if(plan.probs.begin(), plan.probs.end(),
+                                    [](const Prob &pr) { return pr.tb_off == NO_TB_OFF; }));
 
     if (tape) {
         tape->solver = &s;
@@ -911,6 +955,15 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         std::swap(tape->seeds.cap, s.b_seeds.cap);
         std::swap(tape->tb.p, s.b_tb.p);
         std::swap(tape->tb.cap, s.b_tb.cap);
+        tape->ckpt.clear();
+        for (const Prob &pr : plan.probs)
+            if (pr.tb_off == NO_TB_OFF) tape->ckpt.push_back(pr);
+        if (!tape->ckpt.empty()) {
+            std::swap(tape->rowbuf.p, s.b_rowbuf.p);
+            std::swap(tape->rowbuf.cap, s.b_rowbuf.cap);
+            tape->kargs = a;  // a.rowbuf now points into the tape's row buffer
+            tape->band_slots = plan.band_slots;
+        }
         s.last_plan.valid = false;  // its descriptors moved to the tape
         return;
     }
@@ -1375,6 +1428,13 @@ int pa_solver_last_kernel_ms(const pa_solver *s, double *ms, int32_t *launches, 
     });
 }
 
+int pa_solver_set_checkpoint(pa_solver *s, int64_t min_len) {
+    return guard([&] {
+        if (!s) fail(PA_EINVAL, "null solver");
+        s->ckpt_min = min_len;
+    });
+}
+
 int pa_solver_set_graphs(pa_solver *s, int32_t enable) {
     return guard([&] {
         if (!s) fail(PA_EINVAL, "null solver");
@@ -1383,11 +1443,12 @@ int pa_solver_set_graphs(pa_solver *s, int32_t enable) {
     });
 }
 
-int pa_solver_last_schedule(const pa_solver *s, int32_t *groups, int32_t *band_problems) {
+int pa_solver_last_schedule(const pa_solver *s, int32_t *groups, int32_t *band_problems, int32_t *checkpointed) {
     return guard([&] {
         if (!s) fail(PA_EINVAL, "null solver");
         if (groups) *groups = s->last_groups;
         if (band_problems) *band_problems = s->last_band;
+        if (checkpointed) *checkpointed = s->last_ckpt;
     });
 }
 
@@ -1407,6 +1468,129 @@ int pa_forward_ragged(pa_solver *s, const pa_ragged *batch, float *scores_device
         *tape = t.release();
     });
 }
+
+}  // extern "C"
+
+namespace {
+
+// Backward over the checkpointed problems of a tape.  Their forward kept
+// only the bottom row of every strip (band mode, rows in the tape's row
+// buffer).  The walk runs one strip per pass, last strip first: each pass
+// recomputes that strip's one-optimal traceback into a scratch buffer with
+// the band kernel (row above from the kept rows; every wait satisfied by a
+// progress count of "all published") and then continues each problem's
+// walk through it (pa_backward_strip_kernel).  Problems are taken in chunks
+// whose one-strip tracebacks fit the scratch budget.
+void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
+    pa_solver &s = *t.solver;
+    const int bits = tb_bits(t.gap, TR_ONE);
+    const int64_t budget = int64_t(256) << 20;  // scratch traceback words per pass (1 GiB)
+    LaunchSpec spec;
+    std::memset(&spec, 0, sizeof(spec));
+    spec.loc = t.loc;
+    spec.dir = t.dir;
+    spec.gap = t.gap;
+    spec.form = t.form;
+    spec.tr = TR_ONE;
+    spec.R = BAND_R;
+    const int waves_per_cu = std::max(1, occupancy_waves(spec));
+    t.s_progress.ensure(size_t(std::max<int64_t>(t.band_slots, 1)) * 4);
+    t.s_partials.ensure(size_t(std::max<int64_t>(t.band_slots, 1)) * sizeof(BandPartial));
+    t.s_err.ensure(4);
+    t.s_queue.ensure(4);
+    HIPCHK(launch_fill_u32(t.s_progress.as<uint32_t>(), t.band_slots, 0xffffffffu, st));
+    HIPCHK(launch_zero_u32(t.s_err.as<uint32_t>(), 1, st));
+    auto words_of = [&](const Prob &pr) {
+        return (int64_t(pr.len_t) + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
+    };
+    auto strips_of = [&](const Prob &pr) { return (pr.len_s + pr.L * pr.R - 1) / (pr.L * pr.R); };
+    size_t c0 = 0;
+    while (c0 < t.ckpt.size()) {
+        size_t c1 = c0;
+        int64_t words = 0;
+        while (c1 < t.ckpt.size() && (c1 == c0 || words + words_of(t.ckpt[c1]) <= budget)) words += words_of(t.ckpt[c1++]);
+        const int nc = int(c1 - c0);
+        std::vector<int64_t> base_off(static_cast<size_t>(nc));
+        std::vector<StripWalk> init(static_cast<size_t>(nc));
+        int max_nst = 0;
+        int64_t acc = 0;
+        for (int q = 0; q < nc; q++) {
+            const Prob &pr = t.ckpt[c0 + size_t(q)];
+            base_off[size_t(q)] = acc;
+            acc += words_of(pr);
+            max_nst = std::max(max_nst, strips_of(pr));
+            StripWalk w;
+            std::memset(&w, 0, sizeof(w));
+            w.i = pr.len_s - 1;  // the global seed
+            w.j = pr.len_t - 1;
+            w.cur_i = w.cur_j = -1;
+            init[size_t(q)] = w;
+        }
+        t.s_tb.ensure(size_t(std::max<int64_t>(acc, 1)) * 4);
+        upload(st, t.s_state, init.data(), init.size());
+        for (int it = 0; it < max_nst; it++) {
+            std::vector<Prob> probs(t.ckpt.begin() + int64_t(c0), t.ckpt.begin() + int64_t(c1));
+            std::vector<int32_t> strip_of(static_cast<size_t>(nc), -1);
+            std::vector<Group> groups;
+            for (int q = 0; q < nc; q++) {
+                Prob &pr = probs[size_t(q)];
+                const int nst = strips_of(pr);
+                const int sp = nst - 1 - it;
+                if (sp < 0) continue;
+                // the strip's words land at this problem's scratch slot
+                pr.tb_off = base_off[size_t(q)] - int64_t(sp) * words_of(pr);
+                strip_of[size_t(q)] = sp;
+                Group g;
+                std::memset(&g, 0, sizeof(g));
+                g.first = q;
+                g.count = 1;
+                g.L = pr.L;
+                g.lgL = ilog2(pr.L);
+                g.max_lt = pr.len_t;
+                g.max_nst = nst;
+                g.R = pr.R;
+                g.strip = sp;
+                groups.push_back(g);
+            }
+            upload(st, t.s_probs, probs.data(), probs.size());
+            upload(st, t.s_groups, groups.data(), groups.size());
+            upload(st, t.s_strip, strip_of.data(), strip_of.size());
+            HIPCHK(launch_zero_u32(t.s_queue.as<uint32_t>(), 1, st));
+            KArgs k = t.kargs;
+            k.probs = t.s_probs.as<Prob>();
+            k.groups = t.s_groups.as<Group>();
+            k.n_groups = int32_t(groups.size());
+            k.queue = t.s_queue.as<uint32_t>();
+            k.tb = t.s_tb.as<uint32_t>();
+            k.seeds = nullptr;
+            k.scores = nullptr;
+            k.dvals = nullptr;
+            k.val_layers = 1;
+            k.progress = t.s_progress.as<uint32_t>();
+            k.partials = t.s_partials.as<BandPartial>();
+            k.band_err = t.s_err.as<uint32_t>();
+            const int64_t wg = (int64_t(groups.size()) + 3) / 4;
+            spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg, int64_t(s.cus) * std::max(1, waves_per_cu / 4))));
+            HIPCHK(launch_solve(spec, k, st));
+            BwArgs bw = base;
+            bw.probs = t.s_probs.as<Prob>();
+            bw.n = nc;
+            bw.tb = t.s_tb.as<uint32_t>();
+            HIPCHK(launch_backward_strips(t.gap, t.form, t.dim, bw, t.s_state.as<StripWalk>(),
+                                          t.s_strip.as<int32_t>(), st));
+            HIPCHK(hipStreamSynchronize(st));  // the host arrays above are reused
+        }
+        c0 = c1;
+    }
+    uint32_t err = 0;
+    HIPCHK(hipMemcpyAsync(&err, t.s_err.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (err) fail(PA_ERUNTIME, "checkpointed backward: a strip recomputation waited too long");
+}
+
+}  // namespace
+
+extern "C" {
 
 int pa_backward(pa_tape *t, const float *grad_scores_device, float *grad_dense_device,
                 float *grad_series_s_device, float *grad_series_t_device, float *grad_gap_device) {
@@ -1428,8 +1612,10 @@ int pa_backward(pa_tape *t, const float *grad_scores_device, float *grad_dense_d
         a.grad_gap = t->gap == GAP_DTW ? nullptr : grad_gap_device;
         const int form = (t->form == F_TABLE || t->form == F_BINARY) ? F_DENSE : t->form;
         if (t->form == F_TABLE || t->form == F_BINARY) a.grad_dense = nullptr;  // no similarity input to differentiate
-        HIPCHK(launch_backward(t->gap, form, t->dim, a, t->solver->stream));
-        HIPCHK(hipStreamSynchronize(t->solver->stream));
+        hipStream_t st = t->solver->stream;
+        HIPCHK(launch_backward(t->gap, form, t->dim, a, st));
+        if (!t->ckpt.empty()) backward_checkpointed(*t, a, st);
+        HIPCHK(hipStreamSynchronize(st));
     });
 }
 

@@ -20,6 +20,7 @@ enum { TR_SCORE = 0, TR_ONE = 1, TR_ALL = 2 };
 constexpr int WAVE = 64;
 constexpr int MAX_LDS_TABLE = 129;  // K + 1 pad symbol
 constexpr int32_t NO_SEED = INT32_MIN;
+constexpr int64_t NO_TB_OFF = INT64_MIN;  // Prob.tb_off of a problem whose traceback is not kept
 
 // One problem.  Offsets are element offsets into the form's payload.
 struct Prob {
@@ -128,6 +129,22 @@ struct BwArgs {
     float *grad_gap;     // [2 * out]
 };
 hipError_t launch_backward(int gap, int form, int dim, const BwArgs &args, hipStream_t stream);
+
+// Checkpointed backward: one problem's walk continued strip by strip, last
+// strip first, each strip's traceback recomputed just before its walk.
+struct StripWalk {
+    int32_t i, j;      // current cell
+    int32_t mode;      // 0: at a cell; 1: scanning a vertical gap run up column j
+    int32_t pu, pv;    // the cell the vertical run started from
+    int32_t cur_i, cur_j;  // DTW series: rows / columns of the running sums
+    int32_t done;
+    float opens, extends;
+    float gx[4], gy[4];
+};
+// strip_of[q]: the strip problem q walks in this pass (-1: none)
+hipError_t launch_backward_strips(int gap, int form, int dim, const BwArgs &args, StripWalk *state,
+                                  const int32_t *strip_of, hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t value, hipStream_t stream);
 bool supported_R(int form, int tr, bool fastmax, int R);
 int occupancy_waves(const LaunchSpec &spec);  // resident waves per CU for this instantiation
 

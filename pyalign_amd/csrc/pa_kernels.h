@@ -378,9 +378,13 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 }
 
                 if constexpr (TR != TR_SCORE) {
-                    uint32_t *dst = a.tb + pr.tb_off + (int64_t(st) * nsteps_p + s) * int64_t(L) * WPC + l * WPC;
+                    // band-mode problems of a checkpointed forward keep no traceback
+                    if (!BAND || pr.tb_off != NO_TB_OFF) {
+                        uint32_t *dst =
+                            a.tb + pr.tb_off + (int64_t(st) * nsteps_p + s) * int64_t(L) * WPC + l * WPC;
 #pragma unroll
-                    for (int w = 0; w < WPC; ++w) dst[w] = tbw[w];
+                        for (int w = 0; w < WPC; ++w) dst[w] = tbw[w];
+                    }
                 }
 
                 // ---- seeds from the finished column (global / semiglobal) --

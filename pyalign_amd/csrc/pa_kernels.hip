@@ -281,6 +281,7 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_wor
     const int64_t k = int64_t(blockIdx.x) * (blockDim.x / WAVE) + wave;
     if (k >= a.n) return;
     const Prob pr = a.probs[k];
+    if (pr.tb_off == NO_TB_OFF) return;  // checkpointed: walked strip by strip
     const int out = pr.out;
     const float g = a.grad_scores[out];
     const int u0 = a.seeds[2 * out], v0 = a.seeds[2 * out + 1];
@@ -483,15 +484,215 @@ hipError_t launch_band_combine(int loc, int dir, int tr, const Prob *probs, cons
     return hipGetLastError();
 }
 
-__global__ void pa_zero_u32_kernel(uint32_t *p, int64_t n) {
-    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k < n) p[k] = 0u;
+// ---------------------------------------------------------------------------
+// Checkpointed backward: the walk of pa_backward_kernel continued one strip
+// at a time (last strip first), on a traceback recomputed for that strip
+// only.  The walk stops when it leaves the strip upwards and saves where it
+// is, including a vertical gap run still searching for its origin (the
+// only move that crosses strips), the gap counts so far and the DTW
+// running sums; the next pass resumes in the strip above.
+// ---------------------------------------------------------------------------
+template <int GAP, int FORM, int SD>
+__global__ __launch_bounds__(256) void pa_backward_strip_kernel(BwArgs a, StripWalk *state, const int32_t *strip_of,
+                                                                int tile_words) {
+    constexpr int SDD = SD > 0 ? SD : 1;
+    constexpr int WIN = FORM == F_SERIES ? 2 * SERIES_WIN * SDD : 0;
+    extern __shared__ uint32_t smem[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int64_t k = int64_t(blockIdx.x) * (blockDim.x / WAVE) + wave;
+    if (k >= a.n) return;
+    const int st = strip_of[k];
+    if (st < 0) return;
+    StripWalk ws = state[k];
+    if (ws.done) return;
+    const Prob pr = a.probs[k];
+    const int out = pr.out;
+    const float g = a.grad_scores[out];
+    const int lo = st * pr.L * pr.R;
+    uint32_t *mine = smem + size_t(wave) * (tile_words + WIN);
+    TbWalker w;
+    w.init(mine, tile_words, a.tb, pr, GAP == GAP_AFF ? 4 : 2);
+    [[maybe_unused]] RowWin<SDD> wx, wy;
+    if constexpr (FORM == F_SERIES) {
+        float *win = reinterpret_cast<float *>(mine + tile_words);
+        wx.init(win, a.xs + pr.off_s * SD);
+        wy.init(win + SERIES_WIN * SDD, a.xt + pr.off_t * SD);
+    }
+    float opens = ws.opens, extends = ws.extends;
+    [[maybe_unused]] float gx[SDD], gy[SDD];
+#pragma unroll
+    for (int d = 0; d < SDD; ++d) {
+        gx[d] = ws.gx[d];
+        gy[d] = ws.gy[d];
+    }
+    int cur_i = ws.cur_i, cur_j = ws.cur_j;
+    auto flush_row = [&]() {
+        if constexpr (FORM == F_SERIES) {
+            if (cur_i >= 0 && a.grad_xs && lane == 0)
+                for (int d = 0; d < SD; ++d) add_noret(a.grad_xs + (pr.off_s + cur_i) * SD + d, gx[d]);
+        }
+    };
+    auto flush_col = [&]() {
+        if constexpr (FORM == F_SERIES) {
+            if (cur_j >= 0 && a.grad_xt && lane == 0)
+                for (int d = 0; d < SD; ++d) add_noret(a.grad_xt + (pr.off_t + cur_j) * SD + d, gy[d]);
+        }
+    };
+    auto cost_grad = [&](int i, int j) {
+        if constexpr (FORM == F_DENSE) {
+            if (a.grad_dense && lane == 0)
+                add_noret(a.grad_dense + pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj, g);
+        } else if constexpr (FORM == F_SERIES) {
+            if (i != cur_i) {
+                flush_row();
+                cur_i = i;
+                for (int d = 0; d < SD; ++d) gx[d] = 0.0f;
+            }
+            if (j != cur_j) {
+                flush_col();
+                cur_j = j;
+                for (int d = 0; d < SD; ++d) gy[d] = 0.0f;
+            }
+            const float *xr = wx.row(i, lane);
+            const float *yr = wy.row(j, lane);
+#pragma unroll
+            for (int d = 0; d < SD; ++d) {
+                const float gd = g * (2.0f * (xr[d] - yr[d]));
+                gx[d] += gd;
+                gy[d] -= gd;
+            }
+        }
+    };
+    w.seek(ws.i, ws.j);
+    int mode = ws.mode, pu = ws.pu;
+    bool finished = false;
+    while (true) {
+        if (w.i < 0 || w.j < 0) {
+            finished = true;
+            break;
+        }
+        if (w.i < lo) break;  // continues in the strip above
+        if (mode == 1) {      // vertical gap run: up column j to its origin
+            if (w.i > 0 && (w.code(lane) & 4u)) {
+                w.up();
+                continue;
+            }
+            w.up();
+            opens += 1.0f;
+            extends += float(pu - w.i - 1);
+            mode = 0;
+            continue;
+        }
+        const int ci = w.i, cj = w.j;
+        const int src = int(w.code(lane) & 3u);
+        if constexpr (GAP == GAP_DTW) cost_grad(ci, cj);
+        if (src == 0) {  // a local start
+            finished = true;
+            break;
+        }
+        if (src == 1) {
+            if constexpr (GAP != GAP_DTW) cost_grad(ci, cj);
+            w.up();
+            w.left();
+        } else if (src == 2) {
+            if constexpr (GAP == GAP_AFF) {
+                mode = 1;  // the scan starts at this cell
+                pu = ci;
+            } else {
+                opens += 1.0f;
+                w.up();
+            }
+        } else {
+            if constexpr (GAP == GAP_AFF) {
+                while (w.j > 0 && (w.code(lane) & 8u)) w.left();
+                w.left();
+                opens += 1.0f;
+                extends += float(cj - w.j - 1);
+            } else {
+                opens += 1.0f;
+                w.left();
+            }
+        }
+    }
+    if (finished) {
+        flush_row();
+        flush_col();
+        const int u = w.i, v = w.j;
+        if (GAP != GAP_DTW && a.global) {
+            const int run = (u >= 0 && v < 0) ? u + 1 : ((v >= 0 && u < 0) ? v + 1 : 0);
+            if (run > 0) {
+                if constexpr (GAP == GAP_AFF) {
+                    opens += 1.0f;
+                    extends += float(run - 1);
+                } else {
+                    opens += float(run);
+                }
+            }
+        }
+        if (a.grad_gap && lane == 0) {
+            add_noret(a.grad_gap + 2 * out, a.gsign * g * opens);
+            add_noret(a.grad_gap + 2 * out + 1, a.gsign * g * extends);
+        }
+    }
+    if (lane == 0) {
+        ws.i = w.i;
+        ws.j = w.j;
+        ws.mode = mode;
+        ws.pu = pu;
+        ws.cur_i = cur_i;
+        ws.cur_j = cur_j;
+        ws.done = finished ? 1 : 0;
+        ws.opens = opens;
+        ws.extends = extends;
+#pragma unroll
+        for (int d = 0; d < SDD; ++d) {
+            ws.gx[d] = gx[d];
+            ws.gy[d] = gy[d];
+        }
+        state[k] = ws;
+    }
 }
 
-hipError_t launch_zero_u32(uint32_t *p, int64_t n, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(pa_zero_u32_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, p, n);
+hipError_t launch_backward_strips(int gap, int form, int dim, const BwArgs &args, StripWalk *state,
+                                  const int32_t *strip_of, hipStream_t st) {
+    if (args.n == 0) return hipSuccess;
+    const WalkShape ws = walk_shape(args.n);
+    const dim3 grid(unsigned((args.n + ws.waves - 1) / ws.waves)), block(ws.waves * WAVE);
+    const int win = form == F_SERIES ? 2 * SERIES_WIN * dim : 0;
+    const size_t lds = size_t(ws.waves) * (ws.tile_words + win) * 4;
+#define PA_BS(G, F, D)                                                                                         \
+    hipLaunchKernelGGL((pa_backward_strip_kernel<G, F, D>), grid, block, lds, st, args, state, strip_of, \
+                       ws.tile_words)
+    if (form == F_DENSE) {
+        if (gap == GAP_AFF) PA_BS(GAP_AFF, F_DENSE, 0);
+        else if (gap == GAP_LIN) PA_BS(GAP_LIN, F_DENSE, 0);
+        else PA_BS(GAP_DTW, F_DENSE, 0);
+    } else if (form == F_SERIES && gap == GAP_DTW) {
+        switch (dim) {
+            case 1: PA_BS(GAP_DTW, F_SERIES, 1); break;
+            case 2: PA_BS(GAP_DTW, F_SERIES, 2); break;
+            case 3: PA_BS(GAP_DTW, F_SERIES, 3); break;
+            case 4: PA_BS(GAP_DTW, F_SERIES, 4); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef PA_BS
     return hipGetLastError();
 }
+
+__global__ void pa_fill_u32_kernel(uint32_t *p, int64_t n, uint32_t value) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = value;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t value, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pa_fill_u32_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t *p, int64_t n, hipStream_t st) { return launch_fill_u32(p, n, 0u, st); }
 
 }  // namespace pa

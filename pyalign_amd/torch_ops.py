@@ -86,7 +86,7 @@ def _new_solver(kind, direction, locality, gap, gap_open, gap_extend, lim_s, lim
 
 class _AlignScore(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, S, gap_open, gap_extend, len_s, len_t, locality, direction, gap):
+    def forward(ctx, S, gap_open, gap_extend, len_s, len_t, locality, direction, gap, checkpoint):
         if not S.is_cuda or S.dtype != torch.float32 or S.dim() != 3:
             raise ValueError("S must be a float32 CUDA tensor of shape [N, Ls, Lt]")
         S = S.contiguous()
@@ -96,6 +96,7 @@ class _AlignScore(torch.autograd.Function):
         if (ls > Ls).any() or (lt > Lt).any():
             raise ValueError("lengths exceed the padded shape of S")
         solver = _solver("align", direction, locality, gap, float(gap_open), float(gap_extend), ls, lt)
+        solver.set_checkpoint(checkpoint)
         dense_off = np.arange(n, dtype=np.int64) * (Ls * Lt)
         rg, keep = algorithm.make_ragged("dense", ls, lt, dense=S.data_ptr(), dense_off=dense_off,
                                          dense_ld=Lt, on_device=True)
@@ -121,27 +122,29 @@ class _AlignScore(torch.autograd.Function):
                                 None, None, ctypes.c_void_p(dgap.data_ptr())))
         d_open = dgap[:, 0].sum()
         d_ext = dgap[:, 1].sum() if ctx.gap == "affine" else None
-        return dS, d_open, d_ext, None, None, None, None, None
+        return dS, d_open, d_ext, None, None, None, None, None, None
 
 
 def align_score(S, len_s, len_t, gap_open=1.0, gap_extend=0.2, locality="global", direction="maximize",
-                gap="affine"):
+                gap="affine", checkpoint=2048):
     """Optimal alignment scores of N problems given padded similarities S[N, Ls, Lt].
 
     gap_open / gap_extend may be floats or 0-dim tensors (requires_grad for
     gap gradients).  Returns float32 scores [N], identical to the solvers'
-    ``Score`` results on the same inputs.
+    ``Score`` results on the same inputs.  Global pairs whose longer side
+    exceeds ``checkpoint`` keep only strip boundary rows in the forward and
+    are recomputed strip by strip in the backward (-1: never).
     """
     if gap not in ("affine", "linear"):
         raise ValueError(gap)
     go = gap_open if torch.is_tensor(gap_open) else torch.tensor(float(gap_open))
     ge = gap_extend if torch.is_tensor(gap_extend) else torch.tensor(float(gap_extend))
-    return _AlignScore.apply(S, go, ge, len_s, len_t, locality, direction, gap)
+    return _AlignScore.apply(S, go, ge, len_s, len_t, locality, direction, gap, int(checkpoint))
 
 
 class _DTW(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x, y, len_x, len_y, direction):
+    def forward(ctx, x, y, len_x, len_y, direction, checkpoint):
         for t in (x, y):
             if not t.is_cuda or t.dtype != torch.float32 or t.dim() != 3:
                 raise ValueError("series must be float32 CUDA tensors [N, L, dim]")
@@ -156,6 +159,7 @@ class _DTW(torch.autograd.Function):
         if (lx > Lx).any() or (ly > Ly).any():
             raise ValueError("lengths exceed the padded series")
         solver = _solver("dtw", direction, None, None, 0.0, 0.0, lx, ly)
+        solver.set_checkpoint(checkpoint)
         rg, keep = algorithm.make_ragged("series", lx, ly, codes_s=x.data_ptr(), codes_t=y.data_ptr(),
                                          off_s=np.arange(n, dtype=np.int64) * Lx,
                                          off_t=np.arange(n, dtype=np.int64) * Ly, dim=dim, on_device=True)
@@ -177,10 +181,12 @@ class _DTW(torch.autograd.Function):
         torch.cuda.current_stream(grad.device).synchronize()
         check(lib().pa_backward(ctx.tape.h, ctypes.c_void_p(grad.data_ptr()), None,
                                 ctypes.c_void_p(dx.data_ptr()), ctypes.c_void_p(dy.data_ptr()), None))
-        return dx, dy, None, None, None
+        return dx, dy, None, None, None, None
 
 
-def dtw_distance(x, y, len_x, len_y, direction="minimize"):
+def dtw_distance(x, y, len_x, len_y, direction="minimize", checkpoint=2048):
     """DTW distances of N series pairs x[N, Lx, dim], y[N, Ly, dim] (squared
-    Euclidean cost), differentiable with respect to x and y."""
-    return _DTW.apply(x, y, len_x, len_y, direction)
+    Euclidean cost), differentiable with respect to x and y.  Pairs whose
+    longer side exceeds ``checkpoint`` are recomputed strip by strip in the
+    backward instead of keeping their traceback (-1: never)."""
+    return _DTW.apply(x, y, len_x, len_y, direction, int(checkpoint))

@@ -1,0 +1,103 @@
+"""Checkpointed differentiation on the GPU.
+
+Long global pairs keep only strip boundary rows in the forward; the
+backward recomputes one strip's traceback at a time (last strip first) and
+continues the walk through it.  Scores and every gradient must be
+bit-identical to the run that keeps the full traceback (the sums are taken
+in the same order), and the schedule must show the pairs checkpointed.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from pyalign_amd import torch_ops
+from pyalign_amd.torch_ops import align_score, dtw_distance
+
+pytestmark = pytest.mark.gpu
+
+
+def last_schedule():
+    return list(torch_ops._SOLVERS.values())[-1].last_schedule()
+
+
+def walks(rng, n, lo, hi, dim=3):
+    lx = rng.integers(lo, hi + 1, n).astype(np.int32)
+    ly = rng.integers(lo, hi + 1, n).astype(np.int32)
+    X = np.zeros((n, int(lx.max()), dim), np.float32)
+    Y = np.zeros((n, int(ly.max()), dim), np.float32)
+    for k in range(n):
+        X[k, :lx[k]] = np.cumsum(rng.normal(0, 0.1, (lx[k], dim)), 0)
+        Y[k, :ly[k]] = np.cumsum(rng.normal(0, 0.1, (ly[k], dim)), 0)
+    return X, Y, lx, ly
+
+
+def dtw_run(X, Y, lx, ly, checkpoint):
+    Xt = torch.tensor(X, device="cuda", requires_grad=True)
+    Yt = torch.tensor(Y, device="cuda", requires_grad=True)
+    d = dtw_distance(Xt, Yt, lx, ly, checkpoint=checkpoint)
+    sched = last_schedule()
+    d.sum().backward()
+    return d.detach().cpu().numpy(), Xt.grad.cpu().numpy(), Yt.grad.cpu().numpy(), sched
+
+
+@pytest.mark.parametrize("dim", [1, 3])
+def test_dtw_checkpointed_equals_full_traceback(dim):
+    rng = np.random.default_rng(101 + dim)
+    X, Y, lx, ly = walks(rng, 12, 300, 1700, dim)
+    full = dtw_run(X, Y, lx, ly, -1)
+    ck = dtw_run(X, Y, lx, ly, 1000)
+    assert full[3][2] == 0 and ck[3][2] > 0, (full[3], ck[3])
+    for a, b in zip(full[:3], ck[:3]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def align_run(S, ls, lt, checkpoint, gap):
+    St = torch.tensor(S, device="cuda", requires_grad=True)
+    go = torch.tensor(1.0, requires_grad=True)
+    ge = torch.tensor(0.2, requires_grad=True)
+    s = align_score(St, ls, lt, go, ge if gap == "affine" else 0.0, locality="global", gap=gap,
+                    checkpoint=checkpoint)
+    sched = last_schedule()
+    s.sum().backward()
+    out = [s.detach().cpu().numpy(), St.grad.cpu().numpy(), go.grad.numpy().reshape(1)]
+    if gap == "affine":
+        out.append(ge.grad.numpy().reshape(1))
+    return out, sched
+
+
+@pytest.mark.parametrize("gap", ["affine", "linear"])
+def test_align_checkpointed_equals_full_traceback(gap):
+    rng = np.random.default_rng(111)
+    n = 8
+    ls = rng.integers(200, 1400, n).astype(np.int32)
+    lt = rng.integers(200, 1400, n).astype(np.int32)
+    S = (rng.standard_normal((n, int(ls.max()), int(lt.max()))) / 4).astype(np.float32)
+    full, sf = align_run(S, ls, lt, -1, gap)
+    ck, sc = align_run(S, ls, lt, 600, gap)
+    assert sf[2] == 0 and sc[2] > 0, (sf, sc)
+    for a, b in zip(full, ck):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_minimizing_linear_checkpointed():
+    """Global minimisation with a free linear gap over a dense cost, through
+    the checkpointed backward, equals the full-traceback run."""
+    rng = np.random.default_rng(112)
+    n = 4
+    ls = rng.integers(900, 1300, n).astype(np.int32)
+    lt = rng.integers(900, 1300, n).astype(np.int32)
+    C = rng.random((n, int(ls.max()), int(lt.max()))).astype(np.float32)
+
+    def run(ck):
+        Ct = torch.tensor(C, device="cuda", requires_grad=True)
+        s = align_score(Ct, ls, lt, 0.5, 0.0, locality="global", direction="minimize", gap="linear",
+                        checkpoint=ck)
+        sched = last_schedule()
+        s.sum().backward()
+        return (s.detach().cpu().numpy(), Ct.grad.cpu().numpy()), sched
+
+    (a, sa), (b, sb) = run(-1), run(500)
+    assert sa[2] == 0 and sb[2] > 0
+    for x, y in zip(a, b):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
