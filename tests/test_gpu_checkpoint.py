@@ -101,3 +101,33 @@ def test_minimizing_linear_checkpointed():
     assert sa[2] == 0 and sb[2] > 0
     for x, y in zip(a, b):
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def test_dtw_full_size_batch():
+    """8,192 DTW pairs, lengths U[128,4096] (3-D random walks): distances and
+    gradients with checkpointed strips equal the full-traceback run, and two
+    pairs (one checkpointed, one not) match the oracle's optimal path."""
+    import bench
+    import oracle as O
+    import pyalign_amd as pa
+    X, Y, lx, ly = bench.c5_inputs()
+    full = dtw_run(X, Y, lx, ly, -1)
+    ck = dtw_run(X, Y, lx, ly, 2048)
+    assert full[3][2] == 0 and ck[3][2] > 4000
+    for a, b in zip(full[:3], ck[:3]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    d, gx = ck[0], ck[1]
+    pf = pa.series()
+    lims = (pa.solve.SizeRounder()(int(lx.max())), pa.solve.SizeRounder()(int(ly.max())))
+    longest = int(np.argmax(np.maximum(lx, ly)))
+    short = int(np.argmin(np.maximum(lx, ly)))
+    for k in (longest, short):
+        p = pf.new_problem(X[k, :lx[k]], Y[k, :ly[k]])
+        ref = O.OracleResult(O.make_config(solver_type=O.DTW, direction=O.MINIMIZE, goal=O.GOAL_ONE,
+                                           lim_s=lims[0], lim_t=lims[1]), p.matrix)
+        assert np.float32(d[k]).tobytes() == ref.score.tobytes(), k
+        ex = np.zeros_like(X[k, :lx[k]])
+        for u, v in ref.path().tolist():
+            if u >= 0 and v >= 0:
+                ex[u] += 2 * (X[k, u] - Y[k, v])
+        np.testing.assert_allclose(gx[k, :lx[k]], ex, atol=1e-3)
