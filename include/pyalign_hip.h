@@ -92,6 +92,13 @@ void pa_options_free(pa_options *opts);
 int pa_create_solver(int64_t max_len_s, int64_t max_len_t, const pa_options *opts,
                      pa_solver **out);
 void pa_solver_free(pa_solver *solver);
+/* General gap costs (both sides PA_GAP_GENERAL): the cost vectors w(k),
+ * k = 0 .. max_len, that GeneralGapCostSolver's constructor takes from the
+ * gap objects' costs(max_len + 1) (solver.h:3093-3125, factory.h:20-31);
+ * lengths are checked as check_gap_tensor_shape does (solver.h:3047-3054).
+ * Must be called before the first solve of such a solver. */
+int pa_solver_set_gap_costs(pa_solver *solver, const float *w_s, int64_t n_s, const float *w_t,
+                            int64_t n_t);
 
 /* Solver.batch_size (pyalign.h:24, factory.h:422): lanes per call. */
 int pa_solver_batch_size(const pa_solver *solver);

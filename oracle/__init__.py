@@ -20,7 +20,7 @@ ALIGNMENT, DTW = 0, 1
 MINIMIZE, MAXIMIZE = 0, 1
 LOCAL, GLOBAL, SEMIGLOBAL = 0, 1, 2
 GOAL_SCORE, GOAL_ONE, GOAL_ALL = 0, 1, 2
-GAP_LINEAR, GAP_AFFINE = 0, 1
+GAP_LINEAR, GAP_AFFINE, GAP_GENERAL = 0, 1, 2
 
 
 class OcConfig(ctypes.Structure):
@@ -33,6 +33,7 @@ class OcConfig(ctypes.Structure):
         ("aff_u_s", ctypes.c_float), ("aff_v_s", ctypes.c_float),
         ("aff_u_t", ctypes.c_float), ("aff_v_t", ctypes.c_float),
         ("lim_s", ctypes.c_int64), ("lim_t", ctypes.c_int64),
+        ("gen_s", ctypes.c_void_p), ("gen_t", ctypes.c_void_p),
     ]
 
 
@@ -108,7 +109,9 @@ def rounded_limit(x, limit=1024):
 
 def make_config(locality=GLOBAL, direction=MAXIMIZE, goal=GOAL_ONE, gap=("linear", 0.0),
                 solver_type=ALIGNMENT, index_bits=16, remove_dup=True, lim_s=1, lim_t=1):
-    """gap: ("linear", u) | ("affine", open, extend) | None (= linear 0).
+    """gap: ("linear", u) | ("affine", open, extend) | ("general", w_s, w_t)
+    | None (= linear 0).  General costs are the vectors w(k), k = 0 .. lim,
+    of each side (GapCost.costs(lim + 1) as float32).
 
     Affine follows pyalign.gaps.AffineGapCost (u = open - extend in float64,
     v = extend) and the u/v flip of factory.h:48-52 into
@@ -130,6 +133,14 @@ def make_config(locality=GLOBAL, direction=MAXIMIZE, goal=GOAL_ONE, gap=("linear
         v = float(np.float32(float(open_) - float(extend)))
         c.aff_u_s = c.aff_u_t = u
         c.aff_v_s = c.aff_v_t = v
+    elif gap[0] == "general":
+        ws = np.ascontiguousarray(gap[1], dtype=np.float32)
+        wt = np.ascontiguousarray(gap[2], dtype=np.float32)
+        if ws.shape != (lim_s + 1,) or wt.shape != (lim_t + 1,):
+            raise ValueError("general gap costs need lim + 1 entries per side")
+        c.gap_kind = GAP_GENERAL
+        c._keep = (ws, wt)
+        c.gen_s, c.gen_t = ws.ctypes.data, wt.ctypes.data
     else:
         raise ValueError(gap)
     return c

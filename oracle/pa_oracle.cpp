@@ -44,7 +44,7 @@ enum { T_ALIGNMENT = 0, T_DTW = 1 };
 enum { D_MINIMIZE = 0, D_MAXIMIZE = 1 };
 enum { L_LOCAL = 0, L_GLOBAL = 1, L_SEMIGLOBAL = 2 };
 enum { G_SCORE = 0, G_ONE = 1, G_ALL = 2 };
-enum { GAP_LINEAR = 0, GAP_AFFINE = 1 };
+enum { GAP_LINEAR = 0, GAP_AFFINE = 1, GAP_GENERAL = 2 };
 
 }  // namespace
 
@@ -57,7 +57,7 @@ typedef struct oc_config {
     int32_t direction;   // D_MINIMIZE / D_MAXIMIZE
     int32_t locality;    // L_LOCAL / L_GLOBAL / L_SEMIGLOBAL
     int32_t goal;        // G_SCORE / G_ONE / G_ALL
-    int32_t gap_kind;    // GAP_LINEAR / GAP_AFFINE
+    int32_t gap_kind;    // GAP_LINEAR / GAP_AFFINE / GAP_GENERAL
     int32_t index_bits;  // 16 (m16) or 32 (m32)
     int32_t remove_dup;  // !return_dup (options.h:21)
     int32_t reserved;
@@ -65,6 +65,7 @@ typedef struct oc_config {
     float aff_u_s, aff_v_s;              // AffineCost(u=extend, v=open-extend)
     float aff_u_t, aff_v_t;              //   for s and t (factory.h:48-52)
     int64_t lim_s, lim_t;                // solver limits (max_len_s/t)
+    const float *gen_s, *gen_t;          // general: w(k), k = 0 .. lim (lim + 1 floats)
 } oc_config;
 
 }  // extern "C"
@@ -128,6 +129,7 @@ public:
     int32_t no_tb;                    // no_traceback<Index>() (solver.h:182-185)
     float sgn;                        // gap_sgn
     float w1_s = 0, w1_t = 0, ext_s = 0, ext_t = 0;
+    std::vector<float> gen_s, gen_t;  // general gap costs w(k)
 
     std::unique_ptr<float[]> val;     // [layer][i+1][j+1][lane]
     std::unique_ptr<int32_t[]> tbu, tbv;            // traceback_1
@@ -193,6 +195,15 @@ public:
             for (int64_t i = -1; i < cfg.lim_s; i++)
                 for (int64_t j = -1; j < cfg.lim_t; j++) set_val(0, i, j, w);
             set_val(0, -1, -1, 0.0f);
+            return;
+        }
+        if (cfg.gap_kind == GAP_GENERAL) {
+            // GeneralGapCostSolver ctor (solver.h:3114-3123): the column from
+            // w_s * sgn, then the row from w_t * sgn (its corner last)
+            gen_s.assign(cfg.gen_s, cfg.gen_s + ls1);
+            gen_t.assign(cfg.gen_t, cfg.gen_t + lt1);
+            for (int64_t i = 0; i < ls1; i++) set_val(0, i - 1, -1, global ? gen_s[i] * sgn : 0.0f);
+            for (int64_t j = 0; j < lt1; j++) set_val(0, -1, j - 1, global ? gen_t[j] * sgn : 0.0f);
             return;
         }
         if (cfg.gap_kind == GAP_LINEAR) {
@@ -342,7 +353,7 @@ public:
         if (mls > cfg.lim_s || mlt > cfg.lim_t)
             throw std::runtime_error("exceeded_configured_length");  // solver.h:134-152
         for (int b = 0; b < B; b++) { cur_len_s[b] = len_s[b]; cur_len_t[b] = len_t[b]; }
-        if (!trace && cfg.type == T_ALIGNMENT) {
+        if (!trace && cfg.type == T_ALIGNMENT && cfg.gap_kind != GAP_GENERAL) {
             solve_values(pairwise, mls, mlt);
             return;
         }
@@ -385,6 +396,32 @@ public:
                     const float *lf = &val[idx(0, u, v - 1) * B];
                     for (int b = 0; b < B; b++) x[b] = lf[b] + gt;
                     acc.push(x, int32_t(u), int32_t(v - 1));
+                }
+            }
+            return;
+        }
+        if (cfg.gap_kind == GAP_GENERAL) {
+            // GeneralGapCostSolver::solve (solver.h:3145-3183)
+            for (int64_t u = 0; u < mls; u++) {
+                for (int64_t v = 0; v < mlt; v++) {
+                    pairwise(u, v, S);
+                    Acc acc{this, idx(0, u, v) * B};
+                    if (local) acc.push(zero, no_tb, no_tb);
+                    const float *d = &val[idx(0, u - 1, v - 1) * B];
+                    for (int b = 0; b < B; b++) x[b] = d[b] + S[b];
+                    acc.push(x, int32_t(u - 1), int32_t(v - 1));
+                    for (int64_t k = -1; k < u; k++) {
+                        const float *c = &val[idx(0, k, v) * B];
+                        const float w = gen_s[size_t(u - k)] * sgn;
+                        for (int b = 0; b < B; b++) x[b] = c[b] + w;
+                        acc.push(x, int32_t(k), int32_t(v));
+                    }
+                    for (int64_t k = -1; k < v; k++) {
+                        const float *c = &val[idx(0, u, k) * B];
+                        const float w = gen_t[size_t(v - k)] * sgn;
+                        for (int b = 0; b < B; b++) x[b] = c[b] + w;
+                        acc.push(x, int32_t(u), int32_t(k));
+                    }
                 }
             }
             return;

@@ -73,6 +73,8 @@ def _algorithm_for(opts):
         return Algorithm("DTW", "n^2", "n^2")
     if opts.gap_kind == "affine":
         return Algorithm("Gotoh", "n^2", "n^2")
+    if opts.gap_kind == "general":
+        return Algorithm("Waterman-Smith-Beyer", "n^3", "n^2")
     return Algorithm("Needleman-Wunsch" if opts.locality == Locality.GLOBAL else "Smith-Waterman", "n^2", "n^2")
 
 
@@ -107,6 +109,7 @@ class Options:
         desc.index_bits = index_bits
         self.locality = Locality.LOCAL
         self.gap_kind = None
+        self.gap_objects = (None, None)
         if self.solver == Type.ALIGNMENT:
             codomain = options["codomain"]
             self.detail = Detail(int(codomain.detail))
@@ -123,7 +126,13 @@ class Options:
             desc.gap_kind_s, desc.gap_kind_t = ks, kt
             desc.gap_s[0], desc.gap_s[1] = vs
             desc.gap_t[0], desc.gap_t[1] = vt
-            self.gap_kind = "affine" if ks == _lib.PA_GAP_AFFINE and kt == _lib.PA_GAP_AFFINE else "linear"
+            if ks == _lib.PA_GAP_AFFINE and kt == _lib.PA_GAP_AFFINE:
+                self.gap_kind = "affine"
+            elif ks == _lib.PA_GAP_GENERAL and kt == _lib.PA_GAP_GENERAL:
+                self.gap_kind = "general"
+            else:
+                self.gap_kind = "linear"
+            self.gap_objects = (gs, gt)
         else:
             self.detail, self.count = Detail.SOLUTION, Count.ONE
         h = ctypes.c_void_p()
@@ -646,7 +655,17 @@ def _make_module(index_bits):
     def create_solver(max_len_s, max_len_t, options):
         h = ctypes.c_void_p()
         check(lib().pa_create_solver(int(max_len_s), int(max_len_t), options._h, ctypes.byref(h)))
-        return Solver(h, options, int(max_len_s), int(max_len_t))
+        solver = Solver(h, options, int(max_len_s), int(max_len_t))
+        if options.gap_kind == "general":
+            # GeneralGapCostSolver takes costs(max_len + 1) from each side's
+            # gap object, as float32 (factory.h:20-31, solver.h:3093-3125)
+            gs, gt = options.gap_objects
+            ws = np.ascontiguousarray(gs.costs(int(max_len_s) + 1), dtype=np.float32)
+            wt = np.ascontiguousarray(gt.costs(int(max_len_t) + 1), dtype=np.float32)
+            if ws.ndim != 1 or wt.ndim != 1:
+                raise ValueError("gap costs must be one-dimensional")
+            check(lib().pa_solver_set_gap_costs(h, ws.ctypes.data, ws.shape[0], wt.ctypes.data, wt.shape[0]))
+        return solver
 
     mod.create_options = create_options
     mod.create_solver = create_solver

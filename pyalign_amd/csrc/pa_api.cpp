@@ -171,6 +171,10 @@ struct pa_solver {
     DevBuf b_border_s, b_border_t, b_probs, b_groups, b_queue, b_cs, b_ct, b_table, b_dense, b_rowbuf;
     DevBuf b_scores, b_seeds, b_tb, b_path, b_path_off, b_path_len, b_dvals;
     DevBuf b_progress, b_partials, b_band_err, b_band_probs;
+    // general gap costs: w(k) * sgn for k = 0 .. lim (GeneralGapCostSolver's tensors)
+    std::vector<float> ws_sgn, wt_sgn;
+    bool gen_ready = false;
+    DevBuf b_ws, b_wt;
     uint32_t *band_err_host = nullptr;  // pinned: the last band-mode launch's failure flag
     bool band_check = false;            // a band-mode flag is on its way to band_err_host
     // launch sequences replayed from a hipGraph when they repeat
@@ -234,6 +238,8 @@ struct pa_result {
     std::vector<float> border_s, border_t;
     std::vector<Prob> probs;  // indexed by problem
     std::vector<std::unique_ptr<IterState>> iters;
+    // general gap costs: dvals holds WSB_PLANES planes per problem; w(k) * sgn
+    std::vector<float> ws_sgn, wt_sgn;
 };
 
 // A forward pass kept for differentiation: the schedule's problem
@@ -273,6 +279,15 @@ void make_borders(pa_solver &s) {
         return;
     }
     if (o.loc != PA_GLOBAL) return;  // Local/Semiglobal fill 0 (solver.h:2021-2029, 2253-2261)
+    if (o.gap == GAP_GEN) {
+        // w(k) * sgn (solver.h:3114-3123); zeros until the costs are given
+        if (s.gen_ready) {
+            for (int64_t k = 0; k < ns; k++) s.border_s[size_t(k)] = s.ws_sgn[size_t(k)];
+            for (int64_t k = 0; k < nt; k++) s.border_t[size_t(k)] = s.wt_sgn[size_t(k)];
+            s.border_s[0] = s.border_t[0];
+        }
+        return;
+    }
     if (o.gap == GAP_LIN) {
         // arange<index>(0, n) * gap * sgn (solver.h:2806-2814)
         for (int64_t k = 0; k < ns; k++) s.border_s[size_t(k)] = (float(k) * o.lin_s) * sgn;
@@ -639,6 +654,165 @@ void upload(hipStream_t st, DevBuf &buf, const T *src, size_t count) {
     if (count) HIPCHK(hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
 }
 
+// General gap costs (Waterman-Smith-Beyer): one workgroup per problem, no
+// strip schedule.  The kernel leaves D, S and the traceback codes of every
+// cell in device memory; all-optimal enumeration and Solution matrices read
+// them on the host.
+void run_wsb(pa_solver &s, const Input &in, const Job &job, std::unique_ptr<pa_result> *result,
+             float *scores_device) {
+    const pa_options &o = s.opt;
+    const int64_t n = in.n;
+    if (!s.gen_ready) fail(PA_EINVAL, "general gap costs were not given to the solver");
+    hipStream_t st = s.stream;
+    std::vector<Prob> probs(static_cast<size_t>(n));
+    int64_t words = 0;
+    double cells = 0;
+    for (int64_t k = 0; k < n; k++) {
+        Prob &pr = probs[size_t(k)];
+        std::memset(&pr, 0, sizeof(pr));
+        pr.out = int32_t(k);
+        pr.len_s = in.len_s[k];
+        pr.len_t = in.len_t[k];
+        pr.band = -1;
+        if (in.form == PA_FORM_DENSE) {
+            pr.off_s = in.dense_off ? in.dense_off[k] : k;
+            pr.si = in.dense_si_fixed ? in.dense_si_fixed : pr.len_t;
+            pr.sj = in.dense_sj_fixed ? in.dense_sj_fixed : 1;
+        } else {
+            pr.off_s = in.off_s[k];
+            pr.off_t = in.off_t[k];
+        }
+        pr.dv_off = words;
+        words += int64_t(WSB_PLANES) * pr.len_s * pr.len_t;
+        cells += double(pr.len_s) * double(pr.len_t);
+    }
+    upload(st, s.b_probs, probs.data(), probs.size());
+    upload(st, s.b_ws, s.ws_sgn.data(), s.ws_sgn.size());
+    upload(st, s.b_wt, s.wt_sgn.data(), s.wt_sgn.size());
+    const void *codes_s = in.codes_s, *codes_t = in.codes_t;
+    const float *dense = in.dense;
+    if (!in.on_device) {
+        if (in.form == PA_FORM_DENSE) {
+            upload(st, s.b_dense, in.dense, size_t(in.dense_elems));
+            dense = s.b_dense.as<float>();
+        } else {
+            upload(st, s.b_cs, static_cast<const uint8_t *>(in.codes_s), size_t(in.bytes_s));
+            upload(st, s.b_ct, static_cast<const uint8_t *>(in.codes_t), size_t(in.bytes_t));
+            codes_s = s.b_cs.p;
+            codes_t = s.b_ct.p;
+        }
+    }
+    if (job.form == F_TABLE) upload(st, s.b_table, in.table.data(), in.table.size());
+    s.b_dvals.ensure(size_t(std::max<int64_t>(words, 1)) * 4);
+    float *scores = scores_device;
+    if (!scores) {
+        s.b_scores.ensure(size_t(std::max<int64_t>(n, 1)) * 4);
+        scores = s.b_scores.as<float>();
+    }
+    s.b_seeds.ensure(size_t(std::max<int64_t>(n, 1)) * 8);
+
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.probs = s.b_probs.as<Prob>();
+    a.codes_s = codes_s;
+    a.codes_t = codes_t;
+    a.table = s.b_table.as<float>();
+    a.K1 = in.K1;
+    a.eq = in.eq;
+    a.ne = in.ne;
+    a.dense = dense;
+    a.border_s = s.b_border_s.as<float>();
+    a.border_t = s.b_border_t.as<float>();
+    a.scores = scores;
+    a.seeds = s.b_seeds.as<int32_t>();
+    a.dvals = s.b_dvals.as<float>();
+    a.val_layers = 1;
+
+    std::unique_ptr<pa_result> r;
+    int32_t *path = nullptr, *path_len = nullptr;
+    const int64_t *path_off = nullptr;
+    if (result) {
+        r = std::make_unique<pa_result>();
+        if (job.tr == TR_ONE) {
+            r->path_off.resize(size_t(n) + 1);
+            r->path_off[0] = 0;
+            for (int64_t k = 0; k < n; k++)
+                r->path_off[size_t(k) + 1] = r->path_off[size_t(k)] + in.len_s[k] + in.len_t[k] + 2;
+            upload(st, s.b_path_off, r->path_off.data(), r->path_off.size());
+            s.b_path.ensure(size_t(std::max<int64_t>(r->path_off.back(), 1)) * 8);
+            s.b_path_len.ensure(size_t(std::max<int64_t>(n, 1)) * 4);
+            path = s.b_path.as<int32_t>();
+            path_off = s.b_path_off.as<int64_t>();
+            path_len = s.b_path_len.as<int32_t>();
+        }
+    }
+    // workgroup size: 256 threads when the problems fill the CUs, else up to
+    // 1024 (one thread per cell of the longest anti-diagonal) for more loads
+    // in flight per problem
+    int threads = 256;
+    if (n < s.cus) {
+        int64_t diag = 0;
+        for (int64_t k = 0; k < n; k++) diag = std::max<int64_t>(diag, std::min(in.len_s[k], in.len_t[k]));
+        threads = int(std::min<int64_t>(1024, std::max<int64_t>(256, (diag + 63) / 64 * 64)));
+    }
+    HIPCHK(hipEventRecord(s.ev0, st));
+    const hipError_t le = launch_wsb(job.loc, job.dir, job.form, a, n, s.b_ws.as<float>(), s.b_wt.as<float>(),
+                                     path, path_off, path_len, threads, st);
+    if (le != hipSuccess) {
+        std::ostringstream m;
+        m << "kernel launch failed (general gap costs, loc " << job.loc << " dir " << job.dir << " form "
+          << job.form << "): " << hipGetErrorString(le);
+        fail(PA_EHIP, m.str());
+    }
+    HIPCHK(hipEventRecord(s.ev1, st));
+    s.last_launches = 1;
+    s.last_cells = cells;
+    s.last_groups = 0;
+    s.last_band = 0;
+    s.last_ckpt = 0;
+    if (!result) return;  // device scores: caller synchronises
+    r->n = n;
+    r->dir = o.dir;
+    r->loc = job.loc;
+    r->gap = job.gap;
+    r->goal = o.goal;
+    r->index_bits = o.index_bits;
+    r->remove_dup = o.remove_dup;
+    r->len_s.assign(in.len_s, in.len_s + n);
+    r->len_t.assign(in.len_t, in.len_t + n);
+    r->scores.resize(size_t(n));
+    r->seeds.resize(size_t(n) * 2);
+    if (job.tr == TR_ONE) {
+        r->path_len.resize(size_t(n));
+        r->paths.resize(size_t(r->path_off.back()) * 2);
+        HIPCHK(hipMemcpyAsync(r->path_len.data(), path_len, size_t(n) * 4, hipMemcpyDeviceToHost, st));
+        if (!r->paths.empty())
+            HIPCHK(hipMemcpyAsync(r->paths.data(), path, r->paths.size() * 4, hipMemcpyDeviceToHost, st));
+    }
+    if (n) {
+        HIPCHK(hipMemcpyAsync(r->scores.data(), scores, size_t(n) * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(r->seeds.data(), a.seeds, size_t(n) * 8, hipMemcpyDeviceToHost, st));
+    }
+    r->matrices = in.matrices;
+    r->layers = 1;
+    r->tr = job.tr;
+    if (job.tr == TR_ALL || in.matrices) {
+        r->border_s.assign(s.border_s.begin(), s.border_s.end());
+        r->border_t.assign(s.border_t.begin(), s.border_t.end());
+        r->ws_sgn = s.ws_sgn;
+        r->wt_sgn = s.wt_sgn;
+        r->dvals.resize(size_t(words));
+        if (words) HIPCHK(hipMemcpyAsync(r->dvals.data(), a.dvals, size_t(words) * 4, hipMemcpyDeviceToHost, st));
+        r->probs = probs;
+        if (job.tr == TR_ALL) r->iters.resize(size_t(n));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    s.last_ms = ms;
+    *result = std::move(r);
+}
+
 void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, float *scores_device,
          pa_tape *tape = nullptr) {
     ensure_device(s);
@@ -675,6 +849,11 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     if (job.form == F_SERIES && (in.dim < 1 || in.dim > 4)) fail(PA_EINVAL, "series dimension must be 1..4");
     if (tape) job.tr = TR_ONE;  // differentiation walks the one-optimal path
     if (in.matrices && job.tr == TR_SCORE) job.tr = TR_ONE;
+    if (job.gap == GAP_GEN) {
+        if (tape) fail(PA_EINVAL, "differentiation is not available for general gap costs");
+        run_wsb(s, in, job, result, scores_device);
+        return;
+    }
     const int val_layers = in.matrices && job.gap == GAP_AFF ? 3 : 1;
     const bool keep_vals = in.matrices || (job.tr == TR_ALL && job.loc == LOC_LOCAL);
     job.fast = job.form == F_TABLE && job.tr == TR_SCORE && job.loc == LOC_LOCAL && job.dir == DIR_MAX &&
@@ -1108,10 +1287,14 @@ int pa_create_options(const pa_options_desc *d, pa_options **out) {
             o->aff_v_t = float(d->gap_t[0]);
             if (o->aff_u_s < 0 || o->aff_v_s < 0 || o->aff_u_t < 0 || o->aff_v_t < 0)
                 fail(PA_EINVAL, "gap costs must be non-negative");
+        } else if (ks == PA_GAP_GENERAL && kt == PA_GAP_GENERAL) {
+            // GeneralGapCostSolver (factory.h:1126-1133); the cost vectors come
+            // with pa_solver_set_gap_costs
+            o->gap = GAP_GEN;
         } else {
+            // the reference dereferences an empty general cost here (factory.h:1126-1133)
             fail(PA_EINVAL,
-                 "general gap costs (Waterman-Smith-Beyer) are not supported by the MI355X solver; use "
-                 "LinearGapCost or AffineGapCost");
+                 "gap costs of different kinds for s and t: use two linear, two affine or two general costs");
         }
         *out = o.release();
     });
@@ -1425,6 +1608,34 @@ int pa_solver_last_kernel_ms(const pa_solver *s, double *ms, int32_t *launches, 
         if (ms) *ms = s->last_ms;
         if (launches) *launches = s->last_launches;
         if (cells) *cells = s->last_cells;
+    });
+}
+
+int pa_solver_set_gap_costs(pa_solver *s, const float *w_s, int64_t n_s, const float *w_t, int64_t n_t) {
+    return guard([&] {
+        if (!s || !w_s || !w_t) fail(PA_EINVAL, "null argument");
+        if (s->opt.gap != GAP_GEN) fail(PA_EINVAL, "the solver's gap costs are not general");
+        auto check_len = [](int64_t got, int64_t want) {  // check_gap_tensor_shape, solver.h:3041-3045
+            if (got != want) {
+                std::ostringstream e;
+                e << "expected gap cost tensor length of " << want << ", got " << got;
+                fail(PA_EINVAL, e.str());
+            }
+        };
+        check_len(n_s, s->lim_s + 1);
+        check_len(n_t, s->lim_t + 1);
+        const float sgn = s->opt.dir == PA_MINIMIZE ? 1.0f : -1.0f;
+        s->ws_sgn.assign(size_t(n_s), 0.0f);
+        s->wt_sgn.assign(size_t(n_t), 0.0f);
+        for (int64_t k = 0; k < n_s; k++) s->ws_sgn[size_t(k)] = w_s[k] * sgn;
+        for (int64_t k = 0; k < n_t; k++) s->wt_sgn[size_t(k)] = w_t[k] * sgn;
+        s->gen_ready = true;
+        make_borders(*s);
+        if (s->has_device) {
+            HIPCHK(hipMemcpy(s->b_border_s.p, s->border_s.data(), s->border_s.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(s->b_border_t.p, s->border_t.data(), s->border_t.size() * 4, hipMemcpyHostToDevice));
+        }
+        s->last_plan.valid = false;
     });
 }
 
@@ -1744,9 +1955,40 @@ static void push_gap_list(const pa_result *r, const Prob &pr, int32_t i, int32_t
     }
 }
 
+// General gap costs: the cell's value against every candidate, recomputed
+// from the kernel's D and S planes in the same float32 arithmetic
+// (solver.h:3161-3180).  A NaN cell came from a NaN first candidate, which
+// no later candidate replaces or joins.
+static float wsb_D(const pa_result *r, const Prob &pr, int32_t i, int32_t j) {
+    if (i < 0) return r->border_t[size_t(j + 1)];
+    if (j < 0) return r->border_s[size_t(i + 1)];
+    return r->dvals[size_t(pr.dv_off + int64_t(i) * pr.len_t + j)];
+}
+
+static void wsb_cell_edges(const pa_result *r, const Prob &pr, int32_t u, int32_t v, int32_t no_tb,
+                           std::vector<std::pair<int32_t, int32_t>> &out) {
+    const float x = wsb_D(r, pr, u, v);
+    const int64_t cells = int64_t(pr.len_s) * pr.len_t;
+    const float S = r->dvals[size_t(pr.dv_off + 2 * cells + int64_t(u) * pr.len_t + v)];
+    if (std::isnan(x)) {
+        out.emplace_back(u - 1, v - 1);
+        return;
+    }
+    if (r->loc == PA_LOCAL && x == 0.0f) out.emplace_back(no_tb, no_tb);
+    if (wsb_D(r, pr, u - 1, v - 1) + S == x) out.emplace_back(u - 1, v - 1);
+    for (int32_t k = -1; k < u; k++)
+        if (wsb_D(r, pr, k, v) + r->ws_sgn[size_t(u - k)] == x) out.emplace_back(k, v);
+    for (int32_t k = -1; k < v; k++)
+        if (wsb_D(r, pr, u, k) + r->wt_sgn[size_t(v - k)] == x) out.emplace_back(u, k);
+}
+
 static void cell_edges(const pa_result *r, const Prob &pr, int32_t u, int32_t v, int32_t no_tb,
                        std::vector<std::pair<int32_t, int32_t>> &out) {
     out.clear();
+    if (r->gap == GAP_GEN) {
+        wsb_cell_edges(r, pr, u, v, no_tb, out);
+        return;
+    }
     const uint32_t b = tie_bits(r, pr, u, v);
     if (b & 1u) out.emplace_back(no_tb, no_tb);
     if (b & 2u) out.emplace_back(u - 1, v - 1);
@@ -1775,6 +2017,7 @@ int pa_result_iter_next(pa_result *r, int64_t k, float *score) {
         auto Dval = [&](int32_t i, int32_t j) { return r->dvals[size_t(pr.dv_off + int64_t(i) * lt + j)]; };
         auto cont = [&](int32_t u, int32_t v) {
             if (!(u >= 0 && v >= 0)) return false;
+            if (local && r->gap == GAP_GEN) return mx ? Dval(u, v) > 0.0f : Dval(u, v) < 0.0f;
             if (local) return (tie_bits(r, pr, u, v) & 1u) == 0;  // D(u, v) is not zero
             return true;
         };
@@ -1891,6 +2134,29 @@ static void fill_traceback(const pa_result *r, const Prob &pr, int32_t *out) {
     const int64_t ls = pr.len_s, lt = pr.len_t, W = lt + 1, plane = (ls + 1) * W;
     const int32_t no_tb = no_traceback_of(r);
     for (int64_t q = 0; q < r->layers * plane * 2; q++) out[q] = no_tb;
+    if (r->gap == GAP_GEN) {
+        // the kernel's traceback codes: the first candidate holding the value,
+        // which is also the first entry of an all-optimal list
+        const int32_t *code = reinterpret_cast<const int32_t *>(r->dvals.data() + pr.dv_off + 3 * ls * lt);
+        for (int64_t u = 0; u < ls; u++) {
+            for (int64_t v = 0; v < lt; v++) {
+                const int32_t c = code[u * lt + v];
+                int32_t *d = out + 2 * ((u + 1) * W + v + 1);
+                if (c == 0) continue;  // local zero: (no_tb, no_tb)
+                if (c == 1) {
+                    d[0] = int32_t(u - 1);
+                    d[1] = int32_t(v - 1);
+                } else if (c <= u + 2) {
+                    d[0] = c - 3;
+                    d[1] = int32_t(v);
+                } else {
+                    d[0] = int32_t(u);
+                    d[1] = int32_t(c - u - 4);
+                }
+            }
+        }
+        return;
+    }
     const bool aff = r->gap == GAP_AFF;
     const bool all = r->tr == TR_ALL;
     const int bits = all ? 8 : (aff ? 4 : 2);

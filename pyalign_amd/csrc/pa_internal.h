@@ -8,7 +8,7 @@ namespace pa {
 
 enum { LOC_LOCAL = 0, LOC_GLOBAL = 1, LOC_SEMI = 2 };
 enum { DIR_MIN = 0, DIR_MAX = 1 };
-enum { GAP_LIN = 0, GAP_AFF = 1, GAP_DTW = 2 };
+enum { GAP_LIN = 0, GAP_AFF = 1, GAP_DTW = 2, GAP_GEN = 3 };
 // Similarity sources.  F_TABLE reads a (K+1)x(K+1) table (row = s symbol,
 // last symbol = pad whose entries are the worst value) staged in LDS with
 // uint8 codes; F_TABLE_G reads it from global memory with uint16 codes.
@@ -145,6 +145,19 @@ struct StripWalk {
 hipError_t launch_backward_strips(int gap, int form, int dim, const BwArgs &args, StripWalk *state,
                                   const int32_t *strip_of, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t value, hipStream_t stream);
+// General gap costs: one workgroup per problem.  At dvals + dv_off it writes
+// four ls x lt planes: D, D transposed, S and the traceback code of every
+// cell (WSB_* below); then the seed and score, and with `path` set the
+// one-optimal path (as launch_walk writes it).  ws / wt hold w(k) * sgn for
+// k = 0 .. lim.  threads: workgroup size, a multiple of 64 up to 1024.
+// Traceback code of cell (u, v): the first candidate in push order that
+// holds the cell's value: 0 the local zero, 1 the diagonal, 2 + (k + 1) the
+// cell (k, v) above for k = -1 .. u-1, u + 3 + (k + 1) the cell (u, k) to
+// the left for k = -1 .. v-1.
+constexpr int WSB_PLANES = 4;
+hipError_t launch_wsb(int loc, int dir, int form, const KArgs &args, int64_t n, const float *ws, const float *wt,
+                      int32_t *path, const int64_t *path_off, int32_t *path_len, int threads,
+                      hipStream_t stream);
 bool supported_R(int form, int tr, bool fastmax, int R);
 int occupancy_waves(const LaunchSpec &spec);  // resident waves per CU for this instantiation
 

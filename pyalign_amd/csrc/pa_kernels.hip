@@ -682,6 +682,251 @@ hipError_t launch_backward_strips(int gap, int form, int dim, const BwArgs &args
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// General gap costs (GeneralGapCostSolver, solver.h:3047-3184: the
+// Waterman-Smith-Beyer recurrence, O(n m (n + m))).  One workgroup per
+// problem sweeps its anti-diagonals, one cell per thread; every cell folds
+// the reference's candidates in push order with strict improvement:
+// [local zero], the diagonal, the column above from row -1 down
+// (D(k, j) + w_s(i - k)), the row to the left from column -1 on
+// (D(i, k) + w_t(j - k)).  The threads of a wavefront hold consecutive rows
+// i (and columns j = d - i), and scan in lockstep over k: the column scan
+// reads row k of D and the row scan row k of the transposed copy, so both
+// loads are adjacent across the wavefront.  After the sweep the workgroup
+// reduces the seed (solver.h:2042-2057, 2179-2187, 2273-2293) and, for one
+// optimal path, thread 0 follows the traceback codes (build_path,
+// solver.h:858-935).  ws[k] / wt[k] hold w(k) * sgn.
+// ---------------------------------------------------------------------------
+constexpr int WSB_MAX_THREADS = 1024;
+constexpr int WSB_UNROLL = 8;  // candidate loads issued before their compares
+
+// Fold the candidates base[k * stride] + w[top - k], k = 0 .. count-1, into
+// (acc, code) in order, code = code0 + k on a strict improvement.  The loads
+// of WSB_UNROLL candidates go out together; the compares stay in order.
+template <int DIR>
+__device__ __forceinline__ void wsb_scan(const float *base, int64_t stride, const float *w, int top, int count,
+                                         int code0, float &acc, int &code) {
+    int k = 0;
+    for (; k + WSB_UNROLL <= count; k += WSB_UNROLL) {
+        float c[WSB_UNROLL];
+#pragma unroll
+        for (int q = 0; q < WSB_UNROLL; ++q) c[q] = base[int64_t(k + q) * stride] + w[top - k - q];
+#pragma unroll
+        for (int q = 0; q < WSB_UNROLL; ++q) {
+            if (better<DIR>(c[q], acc)) {
+                acc = c[q];
+                code = code0 + k + q;
+            }
+        }
+    }
+    for (; k < count; ++k) {
+        const float c = base[int64_t(k) * stride] + w[top - k];
+        if (better<DIR>(c, acc)) {
+            acc = c;
+            code = code0 + k;
+        }
+    }
+}
+
+template <int LOC, int DIR, int FORM>
+__global__ __launch_bounds__(WSB_MAX_THREADS) void pa_wsb_kernel(KArgs a, const float *ws, const float *wt,
+                                                                 int32_t *path, const int64_t *path_off,
+                                                                 int32_t *path_len) {
+    __shared__ float red_v[WSB_MAX_THREADS];
+    __shared__ int64_t red_p[WSB_MAX_THREADS];
+    const Prob pr = a.probs[blockIdx.x];
+    const int ls = pr.len_s, lt = pr.len_t;
+    const int tid = int(threadIdx.x);
+    if (ls <= 0 || lt <= 0) {
+        if (tid == 0) {
+            write_empty<LOC, DIR>(a, pr);
+            if (path) {
+                const int si = a.seeds[2 * pr.out], sj = a.seeds[2 * pr.out + 1];
+                if (si == NO_SEED) {
+                    path_len[pr.out] = -1;
+                } else {
+                    int32_t *P = path + 2 * path_off[pr.out];
+                    P[0] = si;
+                    P[1] = sj;
+                    path_len[pr.out] = 1;
+                }
+            }
+        }
+        return;
+    }
+    const int64_t cells = int64_t(ls) * lt;
+    float *D = a.dvals + pr.dv_off;
+    float *DT = D + cells;
+    float *SV = D + 2 * cells;
+    int32_t *TB = reinterpret_cast<int32_t *>(D + 3 * cells);
+    auto Dv = [&](int i, int j) -> float {  // D with its borders
+        if (i < 0) return a.border_t[j + 1];
+        if (j < 0) return a.border_s[i + 1];
+        return D[int64_t(i) * lt + j];
+    };
+    for (int d = 0; d <= ls + lt - 2; ++d) {
+        const int i0 = d - lt + 1 > 0 ? d - lt + 1 : 0;
+        const int i1 = d < ls - 1 ? d : ls - 1;
+        for (int i = i0 + tid; i <= i1; i += int(blockDim.x)) {
+            const int j = d - i;
+            float S;
+            if constexpr (FORM == F_DENSE) {
+                S = a.dense[pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj];
+            } else if constexpr (FORM == F_TABLE) {
+                const int x = ((const uint8_t *)a.codes_s)[pr.off_s + i];
+                const int y = ((const uint8_t *)a.codes_t)[pr.off_t + j];
+                S = a.table[x * a.K1 + y];
+            } else {
+                const uint32_t x = ((const uint32_t *)a.codes_s)[pr.off_s + i];
+                const uint32_t y = ((const uint32_t *)a.codes_t)[pr.off_t + j];
+                S = x == y ? a.eq : a.ne;
+            }
+            const float dg = Dv(i - 1, j - 1) + S;
+            float acc;
+            int code;
+            if constexpr (LOC == LOC_LOCAL) {
+                acc = 0.0f;
+                code = 0;
+                if (better<DIR>(dg, acc)) {
+                    acc = dg;
+                    code = 1;
+                }
+            } else {
+                acc = dg;
+                code = 1;
+            }
+            {  // the column above, k = -1 .. i-1
+                const float c = a.border_t[j + 1] + ws[i + 1];
+                if (better<DIR>(c, acc)) {
+                    acc = c;
+                    code = 2;
+                }
+            }
+            wsb_scan<DIR>(D + j, lt, ws, i, i, 3, acc, code);
+            {  // the row to the left, k = -1 .. j-1
+                const float c = a.border_s[i + 1] + wt[j + 1];
+                if (better<DIR>(c, acc)) {
+                    acc = c;
+                    code = i + 3;
+                }
+            }
+            wsb_scan<DIR>(DT + i, ls, wt, j, j, i + 4, acc, code);
+            D[int64_t(i) * lt + j] = acc;
+            DT[int64_t(j) * ls + i] = acc;
+            SV[int64_t(i) * lt + j] = S;
+            TB[int64_t(i) * lt + j] = code;
+        }
+        __syncthreads();
+    }
+
+    // ---- seed: the first optimum in the reference's visiting order ---------
+    // local: rows and columns descending (the largest flat index first);
+    // semiglobal: the last row left to right, then the last column top down
+    // (position p); global: the corner.  Cells that do not beat the worst
+    // value (and NaNs) never seed.
+    constexpr float W = worst<DIR>();
+    int si = ls - 1, sj = lt - 1;
+    float score = D[cells - 1];
+    if constexpr (LOC != LOC_GLOBAL) {
+        float bv = W;
+        int64_t bp = -1;
+        auto fold = [&](float v, int64_t p) {
+            if (!better<DIR>(v, W)) return;
+            const bool earlier = LOC == LOC_LOCAL ? p > bp : p < bp;
+            if (bp < 0 || better<DIR>(v, bv) || (v == bv && earlier)) {
+                bv = v;
+                bp = p;
+            }
+        };
+        if constexpr (LOC == LOC_LOCAL) {
+            for (int64_t q = tid; q < cells; q += blockDim.x) fold(D[q], q);
+        } else {
+            for (int64_t p = tid; p < int64_t(lt) + ls; p += blockDim.x)
+                fold(p < lt ? D[int64_t(ls - 1) * lt + p] : D[(p - lt) * lt + (lt - 1)], p);
+        }
+        red_v[tid] = bv;
+        red_p[tid] = bp;
+        __syncthreads();
+        if (tid == 0) {
+            bv = W;
+            bp = -1;
+            for (int t = 0; t < int(blockDim.x); ++t)
+                if (red_p[t] >= 0) fold(red_v[t], red_p[t]);
+            if (bp < 0) {
+                si = NO_SEED;
+                sj = 0;
+                score = W;
+            } else if (LOC == LOC_LOCAL) {
+                si = int(bp / lt);
+                sj = int(bp % lt);
+                score = bv;
+            } else {
+                si = bp < lt ? ls - 1 : int(bp - lt);
+                sj = bp < lt ? int(bp) : lt - 1;
+                score = bv;
+            }
+        }
+    }
+    if (tid != 0) return;
+    a.scores[pr.out] = score;
+    a.seeds[2 * pr.out] = si;
+    a.seeds[2 * pr.out + 1] = sj;
+    if (!path) return;
+    if (si == NO_SEED) {
+        path_len[pr.out] = -1;
+        return;
+    }
+    // ---- one optimal path: follow the traceback codes ----------------------
+    int32_t *P = path + 2 * path_off[pr.out];
+    int u = si, v = sj, cnt = 1;
+    P[0] = u;
+    P[1] = v;
+    while (u >= 0 && v >= 0) {
+        const int64_t c = int64_t(u) * lt + v;
+        // Local stops where D is not above zero (solver.h:2117-2127)
+        if (LOC == LOC_LOCAL && !better<DIR>(D[c], 0.0f)) break;
+        const int code = TB[c];
+        if (code == 0) break;  // the local zero holds D = 0, stopped above
+        if (code == 1) {
+            u -= 1;
+            v -= 1;
+        } else if (code <= u + 2) {
+            u = code - 3;
+        } else {
+            v = code - u - 4;
+        }
+        P[2 * cnt] = u;
+        P[2 * cnt + 1] = v;
+        ++cnt;
+    }
+    path_len[pr.out] = cnt;
+}
+
+hipError_t launch_wsb(int loc, int dir, int form, const KArgs &a, int64_t n, const float *ws, const float *wt,
+                      int32_t *path, const int64_t *path_off, int32_t *path_len, int threads, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (threads < WAVE || threads > WSB_MAX_THREADS || threads % WAVE) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(n)), block(static_cast<unsigned>(threads));
+#define PA_WSB(LC, DR, FM) \
+    hipLaunchKernelGGL((pa_wsb_kernel<LC, DR, FM>), grid, block, 0, st, a, ws, wt, path, path_off, path_len)
+#define PA_WSB_F(LC, DR)                                        \
+    do {                                                        \
+        if (form == F_DENSE) PA_WSB(LC, DR, F_DENSE);           \
+        else if (form == F_TABLE) PA_WSB(LC, DR, F_TABLE);      \
+        else if (form == F_BINARY) PA_WSB(LC, DR, F_BINARY);    \
+        else return hipErrorInvalidValue;                       \
+    } while (0)
+    const bool mx = dir == DIR_MAX;
+    switch (loc) {
+        case LOC_LOCAL: if (mx) PA_WSB_F(LOC_LOCAL, DIR_MAX); else PA_WSB_F(LOC_LOCAL, DIR_MIN); break;
+        case LOC_GLOBAL: if (mx) PA_WSB_F(LOC_GLOBAL, DIR_MAX); else PA_WSB_F(LOC_GLOBAL, DIR_MIN); break;
+        default: if (mx) PA_WSB_F(LOC_SEMI, DIR_MAX); else PA_WSB_F(LOC_SEMI, DIR_MIN); break;
+    }
+#undef PA_WSB_F
+#undef PA_WSB
+    return hipGetLastError();
+}
+
 __global__ void pa_fill_u32_kernel(uint32_t *p, int64_t n, uint32_t value) {
     const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < n) p[k] = value;

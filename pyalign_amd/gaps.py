@@ -3,9 +3,9 @@
 A gap cost w(k) depends only on the gap length k.  Linear, affine and
 constant-zero costs reduce to the solver's special cases through
 ``to_special_case()`` (gaps.py:13-14, 102-108, 140-143, 181-184); the MI355X
-solver runs those.  The remaining models keep their ``costs(n)`` tables
-(they feed the reference's general Waterman-Smith-Beyer solver, which this
-build does not run; creating a solver with them raises ValueError).
+solver runs those with its O(n m) kernels.  The remaining models give
+their ``costs(n)`` tables to the general Waterman-Smith-Beyer kernel
+(O(n m (n + m)), solver.h:3047-3184).
 Plotting helpers are presentation only and are not provided.
 """
 
@@ -145,8 +145,9 @@ class ExponentialGapCost(GapCost):
         return "exponential", self._u, self._v
 
     def costs(self, n):
-        k = np.arange(n, dtype=np.float64)
-        return (1 - np.power(float(self._u), -(k * self._v))).astype(np.float32)
+        # Python float arithmetic per entry, rounded once to float32
+        # (gaps.py:244-248)
+        return np.array([1 - (self._u ** -(k * self._v)) for k in range(n)], dtype=np.float32)
 
     @property
     def title(self):

@@ -45,11 +45,27 @@ def _opts(**kw):
     return base
 
 
-def test_general_gap_cost_rejected():
-    with pytest.raises(ValueError, match="Waterman-Smith-Beyer"):
-        algorithm.m16.create_options(_opts(gap_cost=pa.LogarithmicGapCost(3, 1)))
-    with pytest.raises(ValueError):
+def test_general_gap_costs_accepted_mixed_kinds_rejected():
+    o = algorithm.m16.create_options(_opts(gap_cost=pa.LogarithmicGapCost(3, 1)))
+    assert o.gap_kind == "general"
+    algorithm.m16.create_options(_opts(gap_cost={"s": pa.ExponentialGapCost(2, 1),
+                                                 "t": pa.LogarithmicGapCost(3, 1)}))
+    # the reference reads an absent general cost here (factory.h:1126-1133)
+    with pytest.raises(ValueError, match="different kinds"):
         algorithm.m16.create_options(_opts(gap_cost={"s": pa.LinearGapCost(1), "t": pa.AffineGapCost(3, 1)}))
+    with pytest.raises(ValueError, match="different kinds"):
+        algorithm.m16.create_options(_opts(gap_cost={"s": pa.LogarithmicGapCost(3, 1), "t": None}))
+
+
+def test_general_gap_cost_vector_length_checked():
+    """check_gap_tensor_shape (solver.h:3047-3054): costs(n) must return n entries."""
+    class Short(pa.gaps.GapCost):
+        def costs(self, n):
+            return np.zeros(n - 1, dtype=np.float32)
+
+    o = algorithm.m16.create_options(_opts(gap_cost=Short()))
+    with pytest.raises(ValueError, match="expected gap cost tensor length of 9, got 8"):
+        algorithm.m16.create_solver(8, 8, o)
 
 
 def test_asymmetric_gap_dict_accepted():
