@@ -58,6 +58,37 @@ def c2_inputs(n=65536, seed=2):
     return table, ls, lt, codes_s, codes_t, off_s, off_t
 
 
+def api_bench(table, ls, lt, codes_s, codes_t, reps=2):
+    """The same bag through the public API: LocalSolver.solve on 65,536
+    alphabetic problems of str sequences (problem objects made before the
+    timed region; joining, device encoding, the solve and the result list
+    inside it)."""
+    import pyalign_amd as pa
+    letters = np.array(list("ACDEFGHIKLMNPQRSTVWY"))
+    byte = np.frombuffer("ACDEFGHIKLMNPQRSTVWY".encode("latin-1"), dtype=np.uint8)
+    text_s = byte[codes_s].tobytes().decode("latin-1")
+    text_t = byte[codes_t].tobytes().decode("latin-1")
+    lookup = {(x, y): float(table[i, j]) for i, x in enumerate(letters) for j, y in enumerate(letters)}
+    pf = pa.alphabetic(list(letters), lambda x, y: lookup[(x, y)])
+    os_ = np.concatenate([[0], np.cumsum(ls.astype(np.int64))])
+    ot = np.concatenate([[0], np.cumsum(lt.astype(np.int64))])
+    t0 = time.perf_counter()
+    probs = [pf.new_problem(text_s[os_[k]:os_[k + 1]], text_t[ot[k]:ot[k + 1]]) for k in range(len(ls))]
+    make_s = time.perf_counter() - t0
+    solver = pa.LocalSolver(gap_cost=pa.AffineGapCost(10, 1), codomain=pa.Score)
+    solver.solve(probs)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        scores = solver.solve(probs)
+    secs = (time.perf_counter() - t0) / reps
+    cells = float((ls.astype(np.float64) * lt).sum())
+    return {"workload": "the batched local Gotoh bag as 65,536 alphabetic str problems through "
+                        "LocalSolver(AffineGapCost(10, 1), codomain=Score).solve (text joined on the host, "
+                        "encoded on the device)",
+            "solve_s": round(secs, 4), "gcups": round(cells / secs / 1e9, 2),
+            "problem_objects_s": round(make_s, 3)}, np.asarray(scores, dtype=np.float32)
+
+
 def c3_inputs(seed=3):
     """One long pair: B uniform over ACGT (10,000), A a 5,000-long window of B
     with 10% substitutions and indel events at p = 0.02 per position (half
@@ -356,8 +387,11 @@ def main():
                "scores_bitwise_equal": bool(np.array_equal(cpu_scores.view(np.uint32),
                                                            auto_scores[:sample].view(np.uint32)))}
 
-    long_pair = fwd_bwd = dtw = None
+    long_pair = fwd_bwd = dtw = api = None
     if not args.no_extra:
+        api, api_scores = api_bench(table, ls, lt, codes_s, codes_t)
+        api["scores_bitwise_equal"] = bool(np.array_equal(api_scores.view(np.uint32), auto_scores.view(np.uint32)))
+        log(f"public API: {api['solve_s']:.3f} s per solve, {api['gcups']:.1f} GCUPS")
         fwd_bwd = c4_bench(5)
         log(f"fwd+bwd: {fwd_bwd['fwd_ms']:.2f} / {fwd_bwd['fwd_bwd_ms']:.2f} ms")
         long_pair = c3_bench(2)
@@ -382,6 +416,7 @@ def main():
         "fwd_bwd": fwd_bwd,
         "long_pair": long_pair,
         "dtw_fwd_bwd": dtw,
+        "public_api": api,
     }
     print(json.dumps(out))
 

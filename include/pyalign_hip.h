@@ -152,6 +152,17 @@ typedef struct pa_ragged {
     const float *dense; /* dense payload */
     const int64_t *dense_off;
     int64_t dense_ld;   /* dense row stride (0: rows of len_t, packed) */
+    /* Problem encoding on the device (no reference counterpart: the
+     * reference encodes each problem in Python, problems/factory.py:103-153,
+     * 205-226).  raw_width 1 or 4: codes_s/codes_t hold raw symbols, 1-byte
+     * (latin-1) or 4-byte (code points), offsets and lengths counted in
+     * symbols.  Indexed form: lut (host, uint8[lut_len]) maps a symbol to its
+     * code, 255 = not in the alphabet (the solve fails with PA_EINVAL "item
+     * not in alphabet" before any DP work; see pa_solver_last_bad_symbol).
+     * Binary form: the symbols are the values (lut = NULL).  0: codes given. */
+    int32_t raw_width;
+    int32_t lut_len;
+    const uint8_t *lut;
 } pa_ragged;
 
 /* Scheduling of a ragged batch. */
@@ -192,6 +203,10 @@ int pa_solver_set_graphs(pa_solver *solver, int32_t enable);
  * last strip first, and continues the walk through it.  Negative: never.
  * No reference counterpart. */
 int pa_solver_set_checkpoint(pa_solver *solver, int64_t min_len);
+/* Where the last solve from raw symbols met symbols outside the alphabet:
+ * the first such position in s's and in t's symbol array, -1 for none.  No
+ * reference counterpart (its encoders raise per problem, s before t). */
+int pa_solver_last_bad_symbol(const pa_solver *solver, int64_t *pos_s, int64_t *pos_t);
 
 /*
  * Differentiation (no reference counterpart: the reference has no

@@ -547,6 +547,13 @@ class Solver:
         check(lib().pa_solver_last_schedule(self._h, ctypes.byref(groups), ctypes.byref(band), ctypes.byref(ckpt)))
         return groups.value, band.value, ckpt.value
 
+    def last_bad_symbol(self):
+        """First positions of symbols outside the alphabet in the last raw
+        solve, (s, t), -1 for none."""
+        ps, pt = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().pa_solver_last_bad_symbol(self._h, ctypes.byref(ps), ctypes.byref(pt)))
+        return ps.value, pt.value
+
     def last_kernel_ms(self):
         ms = ctypes.c_double()
         launches = ctypes.c_int32()
@@ -579,7 +586,14 @@ def _ragged_one(form, len_s, len_t, p):
             return "series", dict(codes_s=xs[os_ * d:(os_ + a) * d].copy(), codes_t=xt[ot * d:(ot + b) * d].copy(),
                                   off_s=[0], off_t=[0], dim=d)
         cs, ct = np.asarray(p["codes_s"]).ravel(), np.asarray(p["codes_t"]).ravel()
-        q = dict(codes_s=cs[os_:os_ + a].copy(), codes_t=ct[ot:ot + b].copy(), off_s=[0], off_t=[0])
+        w = int(p.get("raw_width", 0)) or 1  # raw symbols: offsets count symbols of w bytes
+        if p.get("raw_width", 0):
+            q = dict(codes_s=cs[os_ * w:(os_ + a) * w].copy(), codes_t=ct[ot * w:(ot + b) * w].copy(),
+                     off_s=[0], off_t=[0], raw_width=w)
+            if p.get("lut") is not None:
+                q["lut"] = p["lut"]
+        else:
+            q = dict(codes_s=cs[os_:os_ + a].copy(), codes_t=ct[ot:ot + b].copy(), off_s=[0], off_t=[0])
         if form == "indexed":
             q["table"] = p["table"]
         else:
@@ -594,10 +608,14 @@ _FORMS = {"dense": _lib.PA_FORM_DENSE, "indexed": _lib.PA_FORM_INDEXED,
 
 
 def make_ragged(form, len_s, len_t, codes_s=None, codes_t=None, off_s=None, off_t=None, table=None,
-                eq=0.0, ne=0.0, dense=None, dense_off=None, dense_ld=0, dim=0, on_device=False):
+                eq=0.0, ne=0.0, dense=None, dense_off=None, dense_ld=0, dim=0, on_device=False, raw_width=0,
+                lut=None):
     """Build a pa_ragged descriptor; returns (struct, arrays to keep alive).
 
     With on_device=True the payload arguments are device pointers (ints).
+    raw_width 1 or 4: codes_s/codes_t are raw symbols (bytes of latin-1 or
+    UTF-32-LE text) that the device encodes, through lut for the indexed
+    form.
     """
     keep = []
 
@@ -624,16 +642,25 @@ def make_ragged(form, len_s, len_t, codes_s=None, codes_t=None, off_s=None, off_
     if off_s is not None:
         rg.off_s = arr(off_s, np.int64)
         rg.off_t = arr(off_t, np.int64)
-    if form == "indexed":
+    if raw_width:
+        rg.raw_width = int(raw_width)
         rg.codes_s = payload(codes_s, np.uint8)
         rg.codes_t = payload(codes_t, np.uint8)
+        if lut is not None:
+            rg.lut = arr(lut, np.uint8)
+            rg.lut_len = len(lut)
+    if form == "indexed":
+        if not raw_width:
+            rg.codes_s = payload(codes_s, np.uint8)
+            rg.codes_t = payload(codes_t, np.uint8)
         t = np.ascontiguousarray(table, dtype=np.float32)
         keep.append(t)
         rg.table = t.ctypes.data
         rg.K = t.shape[0]
     elif form == "binary":
-        rg.codes_s = payload(codes_s, np.uint32)
-        rg.codes_t = payload(codes_t, np.uint32)
+        if not raw_width:
+            rg.codes_s = payload(codes_s, np.uint32)
+            rg.codes_t = payload(codes_t, np.uint32)
         rg.eq, rg.ne = float(eq), float(ne)
     elif form == "series":
         rg.codes_s = payload(codes_s, np.float32)

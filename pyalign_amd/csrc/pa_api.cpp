@@ -175,6 +175,9 @@ struct pa_solver {
     std::vector<float> ws_sgn, wt_sgn;
     bool gen_ready = false;
     DevBuf b_ws, b_wt;
+    // raw symbols encoded on the device (pa_ragged.raw_width)
+    DevBuf b_raw_s, b_raw_t, b_lut, b_bad;
+    int64_t last_bad[2] = {-1, -1};  // first symbol outside the alphabet, per side
     uint32_t *band_err_host = nullptr;  // pinned: the last band-mode launch's failure flag
     bool band_check = false;            // a band-mode flag is on its way to band_err_host
     // launch sequences replayed from a hipGraph when they repeat
@@ -644,6 +647,11 @@ struct Input {
     int schedule = PA_SCHED_AUTO;
     bool nan_free = true;
     bool matrices = false;  // keep every layer's cell values (Solution matrices)
+    // raw symbols to encode on the device: 1 or 4 bytes each (0: codes given)
+    int raw_width = 0;
+    const uint8_t *lut = nullptr;  // indexed form: symbol -> code, 255 = none
+    int32_t lut_len = 0;
+    int64_t syms_s = 0, syms_t = 0;  // symbols per side
 };
 
 int goal_to_tr(int goal) { return goal == PA_GOAL_SCORE ? TR_SCORE : (goal == PA_GOAL_ONE ? TR_ONE : TR_ALL); }
@@ -653,6 +661,63 @@ void upload(hipStream_t st, DevBuf &buf, const T *src, size_t count) {
     buf.ensure(std::max<size_t>(count * sizeof(T), 4));
     if (count) HIPCHK(hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
 }
+
+// Device copies of the problem payload (uploaded, or on the device already)
+// and, for raw symbols, their codes.  Fails with ValueError on a symbol
+// outside the alphabet and records where (pa_solver_last_bad_symbol).
+void stage_payload(pa_solver &s, const Input &in, hipStream_t st, const void *&codes_s, const void *&codes_t,
+                   const float *&dense) {
+    codes_s = in.codes_s;
+    codes_t = in.codes_t;
+    dense = in.dense;
+    if (in.form == PA_FORM_DENSE) {
+        if (!in.on_device) {
+            upload(st, s.b_dense, in.dense, size_t(in.dense_elems));
+            dense = s.b_dense.as<float>();
+        }
+        return;
+    }
+    if (!in.raw_width) {
+        if (!in.on_device) {
+            upload(st, s.b_cs, static_cast<const uint8_t *>(in.codes_s), size_t(in.bytes_s));
+            upload(st, s.b_ct, static_cast<const uint8_t *>(in.codes_t), size_t(in.bytes_t));
+            codes_s = s.b_cs.p;
+            codes_t = s.b_ct.p;
+        }
+        return;
+    }
+    const void *raw_s = in.codes_s, *raw_t = in.codes_t;
+    if (!in.on_device) {
+        upload(st, s.b_raw_s, static_cast<const uint8_t *>(in.codes_s), size_t(in.bytes_s));
+        upload(st, s.b_raw_t, static_cast<const uint8_t *>(in.codes_t), size_t(in.bytes_t));
+        raw_s = s.b_raw_s.p;
+        raw_t = s.b_raw_t.p;
+    }
+    const bool indexed = in.form == PA_FORM_INDEXED;
+    const size_t cw = indexed ? 1 : 4;  // code width
+    s.b_cs.ensure(size_t(std::max<int64_t>(in.syms_s, 1)) * cw);
+    s.b_ct.ensure(size_t(std::max<int64_t>(in.syms_t, 1)) * cw);
+    const uint8_t *lut = nullptr;
+    if (indexed) {
+        upload(st, s.b_lut, in.lut, size_t(in.lut_len));
+        lut = s.b_lut.as<uint8_t>();
+    }
+    s.b_bad.ensure(16);
+    unsigned long long *bad = s.b_bad.as<unsigned long long>();
+    HIPCHK(launch_fill_u32(s.b_bad.as<uint32_t>(), 4, 0xffffffffu, st));
+    HIPCHK(launch_translate(raw_s, in.raw_width, in.syms_s, lut, in.lut_len, s.b_cs.p, bad, s.cus, st));
+    HIPCHK(launch_translate(raw_t, in.raw_width, in.syms_t, lut, in.lut_len, s.b_ct.p, bad + 1, s.cus, st));
+    codes_s = s.b_cs.p;
+    codes_t = s.b_ct.p;
+    if (indexed) {
+        unsigned long long first[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(first, bad, sizeof(first), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int side = 0; side < 2; side++) s.last_bad[side] = first[side] == ~0ull ? -1 : int64_t(first[side]);
+        if (s.last_bad[0] >= 0 || s.last_bad[1] >= 0) fail(PA_EINVAL, "item not in alphabet");
+    }
+}
+
 
 // General gap costs (Waterman-Smith-Beyer): one workgroup per problem, no
 // strip schedule.  The kernel leaves D, S and the traceback codes of every
@@ -689,19 +754,9 @@ void run_wsb(pa_solver &s, const Input &in, const Job &job, std::unique_ptr<pa_r
     upload(st, s.b_probs, probs.data(), probs.size());
     upload(st, s.b_ws, s.ws_sgn.data(), s.ws_sgn.size());
     upload(st, s.b_wt, s.wt_sgn.data(), s.wt_sgn.size());
-    const void *codes_s = in.codes_s, *codes_t = in.codes_t;
-    const float *dense = in.dense;
-    if (!in.on_device) {
-        if (in.form == PA_FORM_DENSE) {
-            upload(st, s.b_dense, in.dense, size_t(in.dense_elems));
-            dense = s.b_dense.as<float>();
-        } else {
-            upload(st, s.b_cs, static_cast<const uint8_t *>(in.codes_s), size_t(in.bytes_s));
-            upload(st, s.b_ct, static_cast<const uint8_t *>(in.codes_t), size_t(in.bytes_t));
-            codes_s = s.b_cs.p;
-            codes_t = s.b_ct.p;
-        }
-    }
+    const void *codes_s = nullptr, *codes_t = nullptr;
+    const float *dense = nullptr;
+    stage_payload(s, in, st, codes_s, codes_t, dense);
     if (job.form == F_TABLE) upload(st, s.b_table, in.table.data(), in.table.size());
     s.b_dvals.ensure(size_t(std::max<int64_t>(words, 1)) * 4);
     float *scores = scores_device;
@@ -946,19 +1001,9 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
 
     // ---- device buffers ----------------------------------------------------
     s.b_queue.ensure(std::max<size_t>(plan.launches.size(), 1) * 4);
-    const void *codes_s = in.codes_s, *codes_t = in.codes_t;
-    const float *dense = in.dense;
-    if (!in.on_device) {
-        if (in.form == PA_FORM_DENSE) {
-            upload(st, s.b_dense, in.dense, size_t(in.dense_elems));
-            dense = s.b_dense.as<float>();
-        } else {
-            upload(st, s.b_cs, static_cast<const uint8_t *>(in.codes_s), size_t(in.bytes_s));
-            upload(st, s.b_ct, static_cast<const uint8_t *>(in.codes_t), size_t(in.bytes_t));
-            codes_s = s.b_cs.p;
-            codes_t = s.b_ct.p;
-        }
-    }
+    const void *codes_s = nullptr, *codes_t = nullptr;
+    const float *dense = nullptr;
+    stage_payload(s, in, st, codes_s, codes_t, dense);
     if (job.form == F_TABLE) upload(st, s.b_table, in.table.data(), in.table.size());
     s.b_rowbuf.ensure(size_t(std::max<int64_t>(plan.rowbuf, 1)) * sizeof(float2));
     float *scores = scores_device;
@@ -1509,7 +1554,15 @@ static void ragged_input(const pa_solver *s, const pa_ragged *b, Input &in) {
             if (!b->table) fail(PA_EINVAL, "null table");
             in.bytes_s = end_s;
             in.bytes_t = end_t;
-            if (!b->on_device) {
+            if (b->raw_width) {
+                // codes come from the device translation; its table must map
+                // into the alphabet
+                if (!b->lut || b->lut_len < 1) fail(PA_EINVAL, "raw symbols need a code table");
+                for (int32_t q = 0; q < b->lut_len; q++)
+                    if (b->lut[q] != 255 && b->lut[q] >= b->K) fail(PA_EINVAL, "code table entry beyond the alphabet");
+                in.lut = b->lut;
+                in.lut_len = b->lut_len;
+            } else if (!b->on_device) {
                 // every code indexes the table staged in LDS
                 const uint8_t *cs = static_cast<const uint8_t *>(b->codes_s);
                 const uint8_t *ct = static_cast<const uint8_t *>(b->codes_t);
@@ -1527,6 +1580,7 @@ static void ragged_input(const pa_solver *s, const pa_ragged *b, Input &in) {
             break;
         }
         case PA_FORM_BINARY:
+            if (b->lut) fail(PA_EINVAL, "the binary form takes no code table");
             in.bytes_s = end_s * 4;
             in.bytes_t = end_t * 4;
             in.eq = b->eq;
@@ -1556,6 +1610,16 @@ static void ragged_input(const pa_solver *s, const pa_ragged *b, Input &in) {
             break;
         }
         default: fail(PA_EINVAL, "unknown problem form");
+    }
+    if (b->raw_width) {
+        if (b->raw_width != 1 && b->raw_width != 4) fail(PA_EINVAL, "raw symbols are 1 or 4 bytes wide");
+        if (b->form != PA_FORM_INDEXED && b->form != PA_FORM_BINARY)
+            fail(PA_EINVAL, "raw symbols are for the indexed and binary forms");
+        in.raw_width = b->raw_width;
+        in.syms_s = end_s;
+        in.syms_t = end_t;
+        in.bytes_s = end_s * b->raw_width;
+        in.bytes_t = end_t * b->raw_width;
     }
     in.codes_s = b->codes_s;
     in.codes_t = b->codes_t;
@@ -1636,6 +1700,14 @@ int pa_solver_set_gap_costs(pa_solver *s, const float *w_s, int64_t n_s, const f
             HIPCHK(hipMemcpy(s->b_border_t.p, s->border_t.data(), s->border_t.size() * 4, hipMemcpyHostToDevice));
         }
         s->last_plan.valid = false;
+    });
+}
+
+int pa_solver_last_bad_symbol(const pa_solver *s, int64_t *pos_s, int64_t *pos_t) {
+    return guard([&] {
+        if (!s || !pos_s || !pos_t) fail(PA_EINVAL, "null argument");
+        *pos_s = s->last_bad[0];
+        *pos_t = s->last_bad[1];
     });
 }
 

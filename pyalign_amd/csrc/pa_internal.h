@@ -145,6 +145,10 @@ struct StripWalk {
 hipError_t launch_backward_strips(int gap, int form, int dim, const BwArgs &args, StripWalk *state,
                                   const int32_t *strip_of, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t value, hipStream_t stream);
+// Raw symbols (width 1 or 4 bytes) to codes: uint8 through lut, or uint32
+// values when lut is null; *bad gets the smallest position without a code.
+hipError_t launch_translate(const void *raw, int width, int64_t count, const uint8_t *lut, int32_t lut_len,
+                            void *codes, unsigned long long *bad, int cus, hipStream_t stream);
 // General gap costs: one workgroup per problem.  At dvals + dv_off it writes
 // four ls x lt planes: D, D transposed, S and the traceback code of every
 // cell (WSB_* below); then the seed and score, and with `path` set the

@@ -2,6 +2,8 @@
 // dispatch into the per-locality instantiation units (pa_k_*.hip).
 #include "pa_kernels.h"
 
+#include <algorithm>
+
 namespace pa {
 
 hipError_t dispatch_local_max(const LaunchSpec &, const KArgs &, hipStream_t, int *);
@@ -924,6 +926,44 @@ hipError_t launch_wsb(int loc, int dir, int form, const KArgs &a, int64_t n, con
     }
 #undef PA_WSB_F
 #undef PA_WSB
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Problem encoding on the device: raw symbols of a whole bag (1-byte latin-1
+// or 4-byte code points, the text the host joined) become the solver's codes,
+// through `lut` into uint8 table codes for the indexed form, or widened to
+// uint32 values for the binary form (lut == nullptr).  A symbol without a
+// code (beyond lut_len, or 255 in lut) writes code 0 and leaves the smallest
+// such position in *bad, which the host checks before any DP launch.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pa_translate_kernel(const void *raw, int width, int64_t count,
+                                                           const uint8_t *lut, int32_t lut_len, void *codes,
+                                                           unsigned long long *bad) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < count; q += stride) {
+        const uint32_t x = width == 1 ? uint32_t(static_cast<const uint8_t *>(raw)[q])
+                                      : static_cast<const uint32_t *>(raw)[q];
+        if (lut) {
+            uint32_t c = x < uint32_t(lut_len) ? uint32_t(lut[x]) : 255u;
+            if (c == 255u) {
+                atomicMin(bad, static_cast<unsigned long long>(q));
+                c = 0;
+            }
+            static_cast<uint8_t *>(codes)[q] = uint8_t(c);
+        } else {
+            static_cast<uint32_t *>(codes)[q] = x;
+        }
+    }
+}
+
+hipError_t launch_translate(const void *raw, int width, int64_t count, const uint8_t *lut, int32_t lut_len,
+                            void *codes, unsigned long long *bad, int cus, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    if (width != 1 && width != 4) return hipErrorInvalidValue;
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, int64_t(std::max(cus, 1)) * 16);
+    hipLaunchKernelGGL(pa_translate_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, raw, width,
+                       count, lut, lut_len, codes, bad);
     return hipGetLastError();
 }
 

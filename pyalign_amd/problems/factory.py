@@ -108,6 +108,12 @@ class ByteAlphabetEncoder:
             return r
         out[:len(r)] = r
 
+    def char_lut(self, width):
+        """Code of every 1-byte symbol (255: not in the alphabet), for
+        encoding joined latin-1 text on the device; None for 4-byte text,
+        which this encoder cannot encode."""
+        return self._index if width == 1 and self._encoding == "latin1" else None
+
     def encode_many(self, seqs):
         seqs = [s if isinstance(s, str) else "".join(s) for s in seqs]
         lens = np.fromiter((len(s) for s in seqs), dtype=np.int64, count=len(seqs))
@@ -142,6 +148,24 @@ class AlphabetEncoder:
                 out[i] = ids[x]
         except KeyError as e:
             raise ValueError(f"'{e.args[0]}' is not in alphabet")
+
+    def char_lut(self, width):
+        """Code of every symbol value of `width`-byte text (latin-1 bytes or
+        code points; 255: not in the alphabet) when every alphabet symbol
+        is one character; else None."""
+        if not all(isinstance(x, str) and len(x) == 1 for x in self._alphabet) or len(self._alphabet) > 255:
+            return None
+        points = [ord(x) for x in self._alphabet]
+        if width == 1 and max(points) > 0xFF:
+            points = [p for p in points if p <= 0xFF]
+        size = 256 if width == 1 else max(points) + 1
+        if size > 1 << 21:
+            return None
+        lut = np.full(size, 0xFF, dtype=np.uint8)
+        for x in self._alphabet:
+            if ord(x) < size:
+                lut[ord(x)] = self._ids[x]
+        return lut
 
     def encode_many(self, seqs):
         ids = self._ids

@@ -22,7 +22,7 @@ import numpy as np
 from . import algorithm
 from .gaps import GapCost, ConstantGapCost
 from .problems import ProblemBag, Form
-from .problems.factory import SeriesProblem
+from .problems.factory import AlphabetEncoder, SeriesProblem
 
 
 class Solution:
@@ -417,6 +417,59 @@ def _encode_bag(problems, which):
     return flat, off, lens
 
 
+def _joined_text(problems):
+    """The bag's s and t texts joined (latin-1 bytes when every symbol fits,
+    else UTF-32-LE code points) for encoding on the device, or None when a
+    sequence is not a str."""
+    try:  # join accepts only str items: lists or arrays of symbols go to the host encoders
+        js = "".join([p.s for p in problems])
+        jt = "".join([p.t for p in problems])
+    except TypeError:
+        return None
+    try:
+        return js, jt, js.encode("latin-1"), jt.encode("latin-1"), 1
+    except UnicodeEncodeError:
+        return js, jt, js.encode("utf-32-le"), jt.encode("utf-32-le"), 4
+
+
+def _text_payload(problems, ls, lt, indexed):
+    """Raw-symbol payload of a bag of str problems (the device encodes it):
+    indexed problems need their shared encoder's symbol table (char_lut)."""
+    lut = None
+    enc = getattr(problems[0], "encoder", None)
+    if indexed:
+        if not hasattr(enc, "char_lut") or not all(getattr(p, "encoder", None) is enc for p in problems):
+            return None, None
+    text = _joined_text(problems)
+    if text is None:
+        return None, None
+    js, jt, rs, rt, width = text
+    if indexed:
+        lut = enc.char_lut(width)
+        if lut is None:
+            return None, None
+    payload = dict(codes_s=np.frombuffer(rs, dtype=np.uint8), codes_t=np.frombuffer(rt, dtype=np.uint8),
+                   off_s=_offsets(ls.astype(np.int64)), off_t=_offsets(lt.astype(np.int64)), raw_width=width)
+    if lut is not None:
+        payload["lut"] = lut
+    return payload, (js, jt, enc)
+
+
+def _bad_symbol_error(solver, text, off_s, off_t):
+    """The encoder's error for the first problem (s before t) holding a
+    symbol outside the alphabet (problems/factory.py:103-153)."""
+    js, jt, enc = text
+    ps, pt = solver.last_bad_symbol()
+    cands = []
+    if ps >= 0:
+        cands.append((int(np.searchsorted(off_s, ps, side="right")) - 1, 0, js[ps]))
+    if pt >= 0:
+        cands.append((int(np.searchsorted(off_t, pt, side="right")) - 1, 1, jt[pt]))
+    if not cands or not isinstance(enc, AlphabetEncoder):
+        return ValueError("item not in alphabet")
+    return ValueError(f"'{min(cands)[2]}' is not in alphabet")
+
+
 _KIND = {"for_score": "score", "for_alignment": "alignment", "for_alignment_iterator": "alignment_iterator",
          "for_solution": "solution", "for_solution_iterator": "solution_iterator"}
 
@@ -488,7 +541,21 @@ class Solver:
         with self._timings.measure("prepare"):
             ls = np.array([p.shape[0] for p in problems], dtype=np.int32)
             lt = np.array([p.shape[1] for p in problems], dtype=np.int32)
-            if bag.form == Form.INDEXED_MATRIX_FORM:
+            text = None
+            if bag.form in (Form.INDEXED_MATRIX_FORM, Form.BINARY_MATRIX_FORM) and problems:
+                payload, text = _text_payload(problems, ls, lt, bag.form == Form.INDEXED_MATRIX_FORM)
+            if text is not None:
+                # str sequences: the device encodes the joined text
+                if bag.form == Form.INDEXED_MATRIX_FORM:
+                    table = _shared_table(bag)
+                    if table.shape[0] > 128 or table.shape[1] > 128:
+                        raise ValueError("alphabets above 128 symbols are not supported by the MI355X indexed form")
+                    payload["table"] = np.asarray(table, dtype=np.float32)
+                    form = "indexed"
+                else:
+                    payload["eq"], payload["ne"] = _shared_eq_ne(bag)
+                    form = "binary"
+            elif bag.form == Form.INDEXED_MATRIX_FORM:
                 table = _shared_table(bag)
                 cs, os_, _ = _encode_bag(problems, 0)
                 ct, ot, _ = _encode_bag(problems, 1)
@@ -523,7 +590,12 @@ class Solver:
                 payload = dict(dense=dense, dense_off=dense_off)
                 form = "dense"
         with self._timings.measure("solve"):
-            r = solver.solve_ragged(kind, form, ls, lt, **payload)
+            try:
+                r = solver.solve_ragged(kind, form, ls, lt, **payload)
+            except ValueError as e:
+                if text is None or "not in alphabet" not in str(e):
+                    raise
+                raise _bad_symbol_error(solver, text, payload["off_s"], payload["off_t"]) from None
             return [construct(p, solver, x) for p, x in zip(problems, r)]
 
     def _solve_problem(self, problem):

@@ -166,3 +166,37 @@ def test_gap_cost_tables():
     np.testing.assert_allclose(pa.AffineGapCost(3, 1).costs(4), [0, 3, 4, 5])
     with pytest.raises(RuntimeError):
         pa.AffineGapCost(1, 2)
+
+
+def test_char_lut_matches_encoders():
+    """The device encoder's symbol tables give the host encoders' codes."""
+    from pyalign_amd.problems.factory import AlphabetEncoder, ByteAlphabetEncoder
+    enc = ByteAlphabetEncoder("ACGT", "latin1")
+    lut = enc.char_lut(1)
+    for x in "ACGT":
+        assert lut[ord(x)] == enc.encode(x)[0]
+    assert lut[ord("X")] == 255 and enc.char_lut(4) is None
+    enc = AlphabetEncoder(list("ACDEFGHIKLMNPQRSTVWY"))
+    for width in (1, 4):
+        lut = enc.char_lut(width)
+        for x in "ACDEFGHIKLMNPQRSTVWY":
+            assert lut[ord(x)] == enc.encode([x])[0]
+        assert lut[ord("B")] == 255
+    enc = AlphabetEncoder(list("αβγ"))
+    lut = enc.char_lut(4)
+    assert [lut[ord(x)] for x in "αβγ"] == enc.encode(list("αβγ"))
+    assert AlphabetEncoder(["ab", "c"]).char_lut(4) is None  # multi-character symbols stay on the host
+
+
+def test_text_payload_layout():
+    from pyalign_amd import solve as S
+    pf = pa.alphabetic(list("ACDEFGHIKLMNPQRSTVWY"), lambda x, y: float(x == y))
+    probs = [pf.new_problem("ACD", "EF"), pf.new_problem("", "GHIK"), pf.new_problem("W", "")]
+    ls = np.array([3, 0, 1], np.int32)
+    lt = np.array([2, 4, 0], np.int32)
+    payload, text = S._text_payload(probs, ls, lt, True)
+    assert payload["raw_width"] == 1
+    assert bytes(payload["codes_s"]) == b"ACDW" and bytes(payload["codes_t"]) == b"EFGHIK"
+    assert payload["off_s"].tolist() == [0, 3, 3] and payload["off_t"].tolist() == [0, 2, 6]
+    probs = [pf.new_problem(list("ACD"), "EF")]
+    assert S._text_payload(probs, ls[:1], lt[:1], True) == (None, None)

@@ -1,0 +1,126 @@
+"""Problem encoding on the device: a bag of str problems goes to the GPU as
+its joined text (latin-1 bytes or UTF-32 code points) and a symbol table;
+a kernel turns it into codes.  Results must equal the host-encoded path
+(sequences given as lists of symbols) bit for bit, and a symbol outside the
+alphabet must raise the reference encoder's error for the first offending
+problem (pyalign/problems/factory.py:103-153).
+"""
+
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pyalign_amd as pa
+from pyalign_amd import solve as solve_mod
+from tests.test_gpu_matrices import assert_matrices
+from tests.test_gpu_parity import dense, rounded
+
+pytestmark = pytest.mark.gpu
+
+ALPHABETS = {
+    "byte": "ACGT",                              # ByteAlphabetEncoder (latin-1 str alphabet)
+    "dict20": list("ACDEFGHIKLMNPQRSTVWY"),      # AlphabetEncoder, latin-1 text
+    "greek": list("αβγδεζηθ"),                   # AlphabetEncoder, 4-byte code points
+}
+
+
+def factory(name, rng, direction="maximize"):
+    alphabet = ALPHABETS[name]
+    K = len(alphabet)
+    t = rng.integers(-3, 5, size=(K, K)).astype(np.float32) / 2
+    t = np.triu(t) + np.triu(t, 1).T
+    lookup = {(x, y): t[i, j] for i, x in enumerate(alphabet) for j, y in enumerate(alphabet)}
+    return pa.alphabetic(alphabet, lambda x, y: lookup[(x, y)], direction=direction)
+
+
+def bag(rng, name, n, lo, hi):
+    alphabet = list(ALPHABETS[name])
+    return [("".join(rng.choice(alphabet, int(rng.integers(lo, hi + 1)))),
+             "".join(rng.choice(alphabet, int(rng.integers(lo, hi + 1))))) for _ in range(n)]
+
+
+def no_host_encoding(monkeypatch):
+    def refuse(*a, **k):
+        raise AssertionError("the bag was encoded on the host")
+    monkeypatch.setattr(solve_mod, "_encode_bag", refuse)
+
+
+@pytest.mark.parametrize("name", list(ALPHABETS))
+@pytest.mark.parametrize("codomain", [pa.Score, pa.Solution])
+def test_indexed_text_equals_host_encoding(name, codomain, monkeypatch):
+    rng = np.random.default_rng(zlib.crc32(repr((name, str(codomain), "enc")).encode()))
+    pf = factory(name, rng)
+    seqs = bag(rng, name, 300, 0, 90)
+    solver = pa.LocalSolver(gap_cost=pa.AffineGapCost(3, 1), codomain=codomain)
+    host = solver.solve([pf.new_problem(list(a), list(b)) for a, b in seqs])
+    with monkeypatch.context() as m:
+        no_host_encoding(m)
+        dev = solver.solve([pf.new_problem(a, b) for a, b in seqs])
+    for (a, b), h, d in zip(seqs, host, dev):
+        if codomain is pa.Score:
+            assert np.float32(h).tobytes() == np.float32(d).tobytes(), (a, b)
+        else:
+            assert np.float32(h.score).tobytes() == np.float32(d.score).tobytes(), (a, b)
+            assert np.array_equal(np.asarray(h.path), np.asarray(d.path)), (a, b)
+
+
+def test_indexed_text_against_oracle_and_matrices(monkeypatch):
+    rng = np.random.default_rng(41)
+    pf = factory("greek", rng)
+    seqs = bag(rng, "greek", 40, 1, 30)
+    lims = (rounded(max(len(a) for a, _ in seqs)), rounded(max(len(b) for _, b in seqs)))
+    probs = [pf.new_problem(a, b) for a, b in seqs]
+    with monkeypatch.context() as m:
+        no_host_encoding(m)
+        sols = pa.GlobalSolver(gap_cost=pa.AffineGapCost(2, 0.5), codomain=pa.Solution).solve(probs)
+    for p, s in zip(probs, sols):
+        ref = O.OracleResult(O.make_config(locality=O.GLOBAL, gap=("affine", 2, 0.5), goal=O.GOAL_ONE,
+                                           lim_s=lims[0], lim_t=lims[1]), dense(p))
+        assert np.float32(s.score).tobytes() == ref.score.tobytes()
+        assert np.array_equal(np.asarray(s.path, dtype=np.int32), ref.path())
+    # Solution matrices re-solve one problem from its slice of the raw text
+    p, s = probs[3], sols[3]
+    ref = O.OracleResult(O.make_config(locality=O.GLOBAL, gap=("affine", 2, 0.5), goal=O.GOAL_ONE,
+                                       lim_s=rounded(len(p.s)), lim_t=rounded(len(p.t))), dense(p))
+    one = pa.GlobalSolver(gap_cost=pa.AffineGapCost(2, 0.5), codomain=pa.Solution).solve(p)
+    assert_matrices(one, ref, (p.s, p.t))
+
+
+@pytest.mark.parametrize("text", ["latin", "wide"])
+def test_binary_text_equals_host_encoding(text, monkeypatch):
+    rng = np.random.default_rng(43 if text == "latin" else 44)
+    alphabet = list("ACGT") if text == "latin" else list("ACGT") + ["λ", "Ж"]
+    seqs = [("".join(rng.choice(alphabet, int(rng.integers(0, 70)))),
+             "".join(rng.choice(alphabet, int(rng.integers(0, 70))))) for _ in range(200)]
+    pf = pa.binary(eq=2, ne=-1)
+    solver = pa.SemiglobalSolver(gap_cost=pa.LinearGapCost(1), codomain=pa.Solution)
+    host = solver.solve([pf.new_problem(np.frombuffer(a.encode("utf-32-le"), np.uint32),
+                                        np.frombuffer(b.encode("utf-32-le"), np.uint32)) for a, b in seqs])
+    with monkeypatch.context() as m:
+        no_host_encoding(m)
+        dev = solver.solve([pf.new_problem(a, b) for a, b in seqs])
+    for (a, b), h, d in zip(seqs, host, dev):
+        assert np.float32(h.score).tobytes() == np.float32(d.score).tobytes(), (a, b)
+        assert np.array_equal(np.asarray(h.path), np.asarray(d.path)), (a, b)
+
+
+def test_symbol_outside_alphabet_errors():
+    rng = np.random.default_rng(45)
+    solver = pa.GlobalSolver(gap_cost=pa.LinearGapCost(1), codomain=pa.Score)
+    # byte encoder: the reference's message (factory.py:121-122)
+    pf = factory("byte", rng)
+    with pytest.raises(ValueError, match="^item not in alphabet$"):
+        solver.solve([pf.new_problem("ACGT", "ACGT"), pf.new_problem("ACXT", "AC")])
+    # dict encoder: the first offending problem's symbol, s before t
+    pf = factory("dict20", rng)
+    probs = [pf.new_problem("ACD", "EFG"), pf.new_problem("ACD", "EFBG"), pf.new_problem("AZC", "EF")]
+    with pytest.raises(ValueError, match="^'B' is not in alphabet$"):
+        solver.solve(probs)
+    probs = [pf.new_problem("ACD", "EFG"), pf.new_problem("AOD", "EFBG")]
+    with pytest.raises(ValueError, match="^'O' is not in alphabet$"):
+        solver.solve(probs)
+    # the solver stays usable after the error
+    assert solver.solve([pf.new_problem("ACD", "ACD")])[0] == pytest.approx(
+        solver.solve([pf.new_problem(list("ACD"), list("ACD"))])[0])
