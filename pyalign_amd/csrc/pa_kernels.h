@@ -494,6 +494,20 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
 // the local borders are +0, and the optimum is a plain running maximum
 // (padded rows read the pad symbol, whose worst-valued entries keep them
 // below the optimum).  The column's symbol is loaded one step ahead.
+//
+// Gotoh here carries P' = max(P, 0) and Q' = max(Q, 0) instead of P and Q:
+// with a non-positive extension term e, max(0, max(P, 0) + e) =
+// max(0, P + e), so P'(i) = max(0, D(i-1) + o, P'(i-1) + e) = max(0, P(i))
+// by induction (the same float additions whenever P >= 0, both sides
+// clamped to 0 otherwise), and D = max(0, diag, P, Q) = max3(diag, P', Q'):
+// one v_max3_f32 per layer instead of a separate zero clamp.
+//
+// The left column D(i, j-1) lives in two register banks used alternately
+// by consecutive steps (steps go in pairs): row r+1 still needs the old
+// D(i, j-1) of row r as its diagonal after row r's new value exists, so a
+// single bank would cost a register copy per cell.  Both banks start at the
+// border value and a lane leaves them untouched while it waits for its
+// first column, so whichever bank a step reads holds the border.
 // ---------------------------------------------------------------------------
 template <int GAP, int R>
 __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g, const float *__restrict__ tab,
@@ -528,36 +542,57 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
         const bool act = st < nst;
         const int row0 = st * rps + l * R;
         int rowsym[R];
-        float Dl[R], Ql[R];
+        float DlA[R], DlB[R], Ql[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = row0 + r;
             rowsym[r] = ((act && i < ls) ? int(cs[off_s + i]) : PAD) * K1;
-            Dl[r] = 0.0f;  // local border D(i, -1) = +0 (solver.h:2021-2029)
-            Ql[r] = NEG;   // Q(i, -1)
+            DlA[r] = 0.0f;  // local border D(i, -1) = +0 (solver.h:2021-2029)
+            DlB[r] = 0.0f;
+            Ql[r] = NEG;    // Q(i, -1)
         }
         float botD = 0.0f, botP = NEG;
         float prevD = 0.0f;  // D(row0-1, -1) = +0
+        // global memory is touched only at the top of a step, one step
+        // ahead: the column's symbol and (lane 0) the row above the strip are
+        // loaded for the next step, and the last lane's bottom row of the
+        // previous step is stored, so every load and store has a whole step
+        // of cells to complete before anything waits on it
         int tnext = 0;
         if (act && l == 0 && lt > 0) tnext = int(ct[off_t]);
+        float nxD = 0.0f, nxP = NEG;  // D(-1, j) = +0, P(-1, j) = worst on the first strip
+        if (l == 0 && st > 0 && act && lt > 0) {
+            const float2 v = a.rowbuf[rb_off];
+            nxD = v.x;
+            nxP = v.y;
+        }
+        bool pend = false;  // a bottom row value waiting to be stored
+        float2 pendv = make_float2(0.0f, 0.0f);
+        int pendj = 0;
 
-        for (int s = 0; s < steps; ++s) {
+        // one step: column j = s - l of this lane's R rows, left column read
+        // from Din, this column written to Dout
+        auto step = [&](int s, float (&Din)[R], float (&Dout)[R]) {
             const int j = s - l;
+            const int tsym = tnext;
+            const float rD = nxD, rP = nxP;
+            if (act && j + 1 >= 0 && j + 1 < lt) tnext = int(ct[off_t + j + 1]);
+            if (l == 0 && st > 0 && act && j + 1 < lt) {
+                const float2 v = a.rowbuf[rb_off + j + 1];
+                nxD = v.x;
+                nxP = v.y;
+            }
+            if (pend) {  // after the loads: the next wait on them need not wait for it
+                a.rowbuf[rb_off + pendj] = pendv;
+                pend = false;
+            }
             float inD = shr1(botD), inP = shr1(botP);
             if (l == 0) {
-                if (st == 0) {
-                    inD = 0.0f;  // D(-1, j) = +0
-                    inP = NEG;
-                } else if (act && j < lt) {
-                    const float2 v = a.rowbuf[rb_off + j];
-                    inD = v.x;
-                    inP = v.y;
-                }
+                inD = rD;
+                inP = rP;
             }
             const float Ddiag = prevD;
             prevD = inD;
-            const int tsym = tnext;
-            if (act && j + 1 >= 0 && j + 1 < lt) tnext = int(ct[off_t + j + 1]);
             if (act && j >= 0 && j < lt) {
                 float Dd = Ddiag, Pup = inP;
                 float Gup = inD + go_s;
@@ -567,24 +602,35 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
                     const float dg = Dd + S;
                     float D;
                     if constexpr (GAP == GAP_AFF) {
-                        const float P = fmaxf(Gup, Pup + ge_s);
-                        const float Q = fmaxf(Dl[r] + go_t, Ql[r] + ge_t);
-                        D = fmaxf(fmaxf(fmaxf(dg, 0.0f), P), Q);
-                        Ql[r] = Q;
-                        Pup = P;
+                        const float Pc = fmaxf(fmaxf(Gup, Pup + ge_s), 0.0f);
+                        const float Qc = fmaxf(fmaxf(Din[r] + go_t, Ql[r] + ge_t), 0.0f);
+                        D = fmaxf(fmaxf(dg, Pc), Qc);
+                        Ql[r] = Qc;
+                        Pup = Pc;
                     } else {
-                        D = fmaxf(fmaxf(fmaxf(dg, 0.0f), Gup), Dl[r] + go_t);
+                        D = fmaxf(fmaxf(fmaxf(dg, 0.0f), Gup), Din[r] + go_t);
                     }
                     best = fmaxf(best, D);
-                    Dd = Dl[r];
-                    Dl[r] = D;
+                    Dd = Din[r];
+                    Dout[r] = D;
                     Gup = D + go_s;
                 }
-                botD = Dl[R - 1];
+                botD = Dout[R - 1];
                 botP = Pup;
-                if (l == L - 1 && st + 1 < nst) a.rowbuf[rb_off + j] = make_float2(botD, botP);
+                if (l == L - 1 && st + 1 < nst) {
+                    pend = true;
+                    pendv = make_float2(botD, botP);
+                    pendj = j;
+                }
             }
+        };
+        int s = 0;
+        for (; s + 1 < steps; s += 2) {
+            step(s, DlA, DlB);
+            step(s + 1, DlB, DlA);
         }
+        if (s < steps) step(s, DlA, DlB);
+        if (pend) a.rowbuf[rb_off + pendj] = pendv;
         __threadfence_block();
     }
     for (int m = 1; m < L; m <<= 1) best = fmaxf(best, __shfl_xor(best, m, WAVE));
