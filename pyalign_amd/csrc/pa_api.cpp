@@ -182,6 +182,8 @@ struct pa_solver {
     bool band_check = false;            // a band-mode flag is on its way to band_err_host
     // launch sequences replayed from a hipGraph when they repeat
     bool use_graphs = true;
+    // fast kernel: single-symbol LDS table even where the pair table fits
+    bool no_pair = std::getenv("PYALIGN_AMD_NO_PAIR_TABLE") != nullptr;
     // a tape's global pairs longer than this (max of the two lengths) keep
     // only strip boundary rows and are recomputed strip by strip in the
     // backward; negative: never
@@ -1074,8 +1076,15 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         spec.R = R;
         spec.fastmax = fastmax;
         spec.lds_bytes = job.form == F_TABLE ? size_t(in.K1) * in.K1 * 4 : 0;
+        if (fastmax && R == 0 && in.K1 >= 2) {
+            const size_t pair_bytes = size_t(in.K1) * in.K1 * (in.K1 - 1) * 8;
+            if (pair_bytes <= PAIR_LDS_MAX && !s.no_pair) {
+                spec.pair = 1;
+                spec.lds_bytes = pair_bytes;
+            }
+        }
         const std::vector<int> okey = {spec.loc, spec.dir, spec.gap, spec.form, spec.tr, spec.R,
-                                       int(spec.fastmax), int(spec.lds_bytes)};
+                                       int(spec.fastmax), int(spec.lds_bytes), spec.pair};
         int waves_per_cu = 0;
         for (auto &e : s.occ_cache)
             if (e.first == okey) waves_per_cu = e.second;

@@ -95,7 +95,7 @@ struct KArgs {
 // R = 0 selects the kernel that serves every strip height in FAST_RS from
 // one queue (the group carries its R).
 #ifndef PA_FAST_RS_LIST
-#define PA_FAST_RS_LIST 8, 16, 24, 32
+#define PA_FAST_RS_LIST 8, 12, 16, 20, 24, 28, 32
 #endif
 constexpr int FAST_RS[] = {PA_FAST_RS_LIST};
 
@@ -104,7 +104,10 @@ struct LaunchSpec {
     bool fastmax;   // v_max_f32 is exact (local maximize score, NaN-free table)
     int grid;       // workgroups of 256 threads (persistent)
     size_t lds_bytes;
+    int pair;       // fast kernel: LDS table of row-symbol pairs (K1^2 * (K1-1) float2)
 };
+// largest pair table the fast kernel stages (two workgroups per CU still fit)
+constexpr size_t PAIR_LDS_MAX = 80 * 1024;
 
 hipError_t launch_solve(const LaunchSpec &spec, const KArgs &args, hipStream_t stream);
 hipError_t launch_walk(int gap, const Prob *probs, int64_t n, const uint32_t *tb, const int32_t *seeds,

@@ -508,8 +508,13 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
 // single bank would cost a register copy per cell.  Both banks start at the
 // border value and a lane leaves them untouched while it waits for its
 // first column, so whichever bank a step reads holds the border.
+//
+// PAIR: the LDS holds the table for row-symbol pairs, entry ((x1 * K1 + x2)
+// * Kc + y) = (S(x1, y), S(x2, y)) with Kc = K1 - 1 real column symbols, so
+// rows 2q and 2q+1 of a lane get their similarities from one ds_read_b64 at
+// one address add (K1^2 * Kc * 8 bytes: 70,560 for 20 symbols and the pad).
 // ---------------------------------------------------------------------------
-template <int GAP, int R>
+template <int GAP, bool PAIR, int R>
 __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g, const float *__restrict__ tab,
                                                  int lane) {
     constexpr float NEG = -__builtin_inff();
@@ -542,6 +547,7 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
         const bool act = st < nst;
         const int row0 = st * rps + l * R;
         int rowsym[R];
+        [[maybe_unused]] int rowpair[R / 2 > 0 ? R / 2 : 1];
         float DlA[R], DlB[R], Ql[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -550,6 +556,11 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
             DlA[r] = 0.0f;  // local border D(i, -1) = +0 (solver.h:2021-2029)
             DlB[r] = 0.0f;
             Ql[r] = NEG;    // Q(i, -1)
+        }
+        if constexpr (PAIR) {
+            // byte offset of the (row 2q, row 2q+1) symbol pair's block
+#pragma unroll
+            for (int q = 0; q < R / 2; ++q) rowpair[q] = (rowsym[2 * q] + rowsym[2 * q + 1] / K1) * (K1 - 1) * 8;
         }
         float botD = 0.0f, botP = NEG;
         float prevD = 0.0f;  // D(row0-1, -1) = +0
@@ -596,9 +607,22 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
             if (act && j >= 0 && j < lt) {
                 float Dd = Ddiag, Pup = inP;
                 float Gup = inD + go_s;
+                [[maybe_unused]] float Sv[R];
+                if constexpr (PAIR) {
+                    const int t8 = tsym * 8;
+#pragma unroll
+                    for (int q = 0; q < R / 2; ++q) {
+                        const float2 v = *reinterpret_cast<const float2 *>(
+                            reinterpret_cast<const char *>(tab) + rowpair[q] + t8);
+                        Sv[2 * q] = v.x;
+                        Sv[2 * q + 1] = v.y;
+                    }
+                }
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const float S = tab[rowsym[r] + tsym];
+                    float S;
+                    if constexpr (PAIR) S = Sv[r];
+                    else S = tab[rowsym[r] + tsym];
                     const float dg = Dd + S;
                     float D;
                     if constexpr (GAP == GAP_AFF) {
@@ -637,11 +661,22 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
     if (has && l == 0) a.scores[out] = (ls == 0 || lt == 0) ? NEG : best;
 }
 
-template <int GAP, int... RS>
+template <int GAP, bool PAIR, int... RS>
 __global__ __launch_bounds__(256) void pa_fast_kernel(KArgs a) {
     extern __shared__ float lds_table[];
-    const int n = a.K1 * a.K1;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) lds_table[k] = a.table[k];
+    const int K1 = a.K1;
+    if constexpr (PAIR) {
+        const int Kc = K1 - 1;
+        const int n = K1 * K1 * Kc;
+        float2 *pt = reinterpret_cast<float2 *>(lds_table);
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            const int y = k % Kc, p = k / Kc;
+            pt[k] = make_float2(a.table[(p / K1) * K1 + y], a.table[(p % K1) * K1 + y]);
+        }
+    } else {
+        const int n = K1 * K1;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) lds_table[k] = a.table[k];
+    }
     __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
     while (true) {
@@ -651,21 +686,26 @@ __global__ __launch_bounds__(256) void pa_fast_kernel(KArgs a) {
         if (t >= uint32_t(a.n_groups)) break;
         const Group &g = a.groups[t];
         const int R = g.R;
-        (void)((R == RS ? (solve_group_fast<GAP, RS>(a, g, lds_table, lane), true) : false) || ...);
+        (void)((R == RS ? (solve_group_fast<GAP, PAIR, RS>(a, g, lds_table, lane), true) : false) || ...);
     }
 }
 
-template <int GAP>
-static hipError_t go_fast(const LaunchSpec &s, const KArgs &a, hipStream_t st, int *occ_out) {
-    const void *fn = reinterpret_cast<const void *>(&pa_fast_kernel<GAP, PA_FAST_RS_LIST>);
+template <int GAP, bool PAIR>
+static hipError_t go_fast_t(const LaunchSpec &s, const KArgs &a, hipStream_t st, int *occ_out) {
+    const void *fn = reinterpret_cast<const void *>(&pa_fast_kernel<GAP, PAIR, PA_FAST_RS_LIST>);
     if (occ_out) {
         int blocks = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 256, s.lds_bytes) != hipSuccess) blocks = 1;
         *occ_out = blocks * 4;
         return hipSuccess;
     }
-    hipLaunchKernelGGL((pa_fast_kernel<GAP, PA_FAST_RS_LIST>), dim3(s.grid), dim3(256), s.lds_bytes, st, a);
+    hipLaunchKernelGGL((pa_fast_kernel<GAP, PAIR, PA_FAST_RS_LIST>), dim3(s.grid), dim3(256), s.lds_bytes, st, a);
     return hipGetLastError();
+}
+
+template <int GAP>
+static hipError_t go_fast(const LaunchSpec &s, const KArgs &a, hipStream_t st, int *occ_out) {
+    return s.pair ? go_fast_t<GAP, true>(s, a, st, occ_out) : go_fast_t<GAP, false>(s, a, st, occ_out);
 }
 
 template <int LOC, int DIR, int GAP, int FORM, int TR, int R, bool FAST, int SD>
