@@ -1092,8 +1092,12 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             waves_per_cu = std::max(1, occupancy_waves(spec));
             s.occ_cache.push_back({okey, waves_per_cu});
         }
-        const int64_t wg_needed = (int64_t(g1 - g0) + 3) / 4;
-        spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg_needed, int64_t(s.cus) * std::max(1, waves_per_cu / 4))));
+        // the fast kernel's large workgroups only where the groups fill every
+        // CU with them; fewer groups spread over more, smaller workgroups
+        spec.block = fastmax && R == 0 && int64_t(g1 - g0) >= int64_t(s.cus) * (FAST_BLOCK / WAVE) ? FAST_BLOCK : 256;
+        const int wpb = spec.block / WAVE;  // waves per workgroup
+        const int64_t wg_needed = (int64_t(g1 - g0) + wpb - 1) / wpb;
+        spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg_needed, int64_t(s.cus) * std::max(1, waves_per_cu / wpb))));
         KArgs la = a;
         la.groups = a.groups + g0;
         la.n_groups = g1 - g0;

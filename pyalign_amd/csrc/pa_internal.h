@@ -105,9 +105,17 @@ struct LaunchSpec {
     int grid;       // workgroups of 256 threads (persistent)
     size_t lds_bytes;
     int pair;       // fast kernel: LDS table of row-symbol pairs (K1^2 * (K1-1) float2)
+    int block;      // threads per workgroup
 };
 // largest pair table the fast kernel stages (two workgroups per CU still fit)
 constexpr size_t PAIR_LDS_MAX = 80 * 1024;
+// fast kernel workgroup: its twelve waves share one LDS table, and at up to
+// 168 VGPRs three waves per SIMD stay resident (768 measured 12% faster than
+// 256-thread groups, which the 70 KB pair table held to two waves per SIMD)
+#ifndef PA_FAST_BLOCK
+#define PA_FAST_BLOCK 768
+#endif
+constexpr int FAST_BLOCK = PA_FAST_BLOCK;
 
 hipError_t launch_solve(const LaunchSpec &spec, const KArgs &args, hipStream_t stream);
 hipError_t launch_walk(int gap, const Prob *probs, int64_t n, const uint32_t *tb, const int32_t *seeds,

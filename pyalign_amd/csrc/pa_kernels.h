@@ -662,7 +662,7 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
 }
 
 template <int GAP, bool PAIR, int... RS>
-__global__ __launch_bounds__(256) void pa_fast_kernel(KArgs a) {
+__global__ __launch_bounds__(FAST_BLOCK) void pa_fast_kernel(KArgs a) {
     extern __shared__ float lds_table[];
     const int K1 = a.K1;
     if constexpr (PAIR) {
@@ -695,11 +695,13 @@ static hipError_t go_fast_t(const LaunchSpec &s, const KArgs &a, hipStream_t st,
     const void *fn = reinterpret_cast<const void *>(&pa_fast_kernel<GAP, PAIR, PA_FAST_RS_LIST>);
     if (occ_out) {
         int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, 256, s.lds_bytes) != hipSuccess) blocks = 1;
-        *occ_out = blocks * 4;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, FAST_BLOCK, s.lds_bytes) != hipSuccess)
+            blocks = 1;
+        *occ_out = blocks * (FAST_BLOCK / WAVE);
         return hipSuccess;
     }
-    hipLaunchKernelGGL((pa_fast_kernel<GAP, PAIR, PA_FAST_RS_LIST>), dim3(s.grid), dim3(256), s.lds_bytes, st, a);
+    const int block = s.block > 0 && s.block <= FAST_BLOCK && s.block % WAVE == 0 ? s.block : 256;
+    hipLaunchKernelGGL((pa_fast_kernel<GAP, PAIR, PA_FAST_RS_LIST>), dim3(s.grid), dim3(block), s.lds_bytes, st, a);
     return hipGetLastError();
 }
 
