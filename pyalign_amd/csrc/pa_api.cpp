@@ -177,6 +177,7 @@ struct pa_solver {
     DevBuf b_ws, b_wt;
     // raw symbols encoded on the device (pa_ragged.raw_width)
     DevBuf b_raw_s, b_raw_t, b_lut, b_bad;
+    DevBuf b_safe, b_sink;  // KArgs::safe, KArgs::sink
     int64_t last_bad[2] = {-1, -1};  // first symbol outside the alphabet, per side
     uint32_t *band_err_host = nullptr;  // pinned: the last band-mode launch's failure flag
     bool band_check = false;            // a band-mode flag is on its way to band_err_host
@@ -334,6 +335,14 @@ void ensure_device(pa_solver &s) {
     *s.band_err_host = 0;
     s.b_border_s.ensure(s.border_s.size() * 4);
     s.b_border_t.ensure(s.border_t.size() * 4);
+    {
+        float safe[SAFE_FLOATS] = {};
+        safe[0] = -std::numeric_limits<float>::infinity();
+        safe[1] = std::numeric_limits<float>::infinity();
+        s.b_safe.ensure(sizeof(safe));
+        HIPCHK(hipMemcpy(s.b_safe.p, safe, sizeof(safe), hipMemcpyHostToDevice));
+        s.b_sink.ensure(SINK_WORDS * 4);
+    }
     HIPCHK(hipMemcpy(s.b_border_s.p, s.border_s.data(), s.border_s.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(s.b_border_t.p, s.border_t.data(), s.border_t.size() * 4, hipMemcpyHostToDevice));
     s.has_device = true;
@@ -780,6 +789,7 @@ void run_wsb(pa_solver &s, const Input &in, const Job &job, std::unique_ptr<pa_r
     a.dense = dense;
     a.border_s = s.b_border_s.as<float>();
     a.border_t = s.b_border_t.as<float>();
+    a.safe = s.b_safe.as<float>();
     a.scores = scores;
     a.seeds = s.b_seeds.as<int32_t>();
     a.dvals = s.b_dvals.as<float>();
@@ -1036,6 +1046,8 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     a.rowbuf = s.b_rowbuf.as<float2>();
     a.border_s = s.b_border_s.as<float>();
     a.border_t = s.b_border_t.as<float>();
+    a.safe = s.b_safe.as<float>();
+    a.sink = s.b_sink.as<uint32_t>();
     a.go_s = s.go_s;
     a.ge_s = s.ge_s;
     a.go_t = s.go_t;

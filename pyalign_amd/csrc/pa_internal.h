@@ -90,7 +90,14 @@ struct KArgs {
     uint32_t *progress;     // band mode: published columns per strip
     BandPartial *partials;  // band mode: seed search per strip
     uint32_t *band_err;     // band mode: set when a wait gave up
+    // SAFE_FLOATS readable floats: [0] = -inf, [1] = +inf, the rest 0; the
+    // address a lane or row with nothing to read loads from
+    const float *safe;
+    // SINK_WORDS writable words: where a lane with nothing to store stores
+    uint32_t *sink;
 };
+constexpr int SAFE_FLOATS = 16;
+constexpr int SINK_WORDS = 16;
 
 // R = 0 selects the kernel that serves every strip height in FAST_RS from
 // one queue (the group carries its R).

@@ -140,9 +140,14 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
 
         int rowsym[R];
         bool rv[R];
-        int64_t rowbase[R];
+        [[maybe_unused]] int64_t rowbase[R];
         float sa[R][SDD];
-        float Dl[R], Ql[R];
+        float DlA[R], DlB[R], Ql[R];
+        // Every load of the step loop is unconditional: a lane or row with
+        // nothing to read loads a.safe (valid memory; its worst value for a
+        // padded row) and a column index past the problem is clamped, so no
+        // load sits behind a branch.
+        const float *safeW = a.safe + (DIR == DIR_MIN ? 1 : 0);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = row0 + r;
@@ -154,80 +159,130 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 const uint32_t *cs = (const uint32_t *)a.codes_s;
                 rowsym[r] = rv[r] ? int(cs[pr.off_s + i]) : 0;
             } else if constexpr (FORM == F_DENSE) {
-                rowbase[r] = pr.off_s + int64_t(rv[r] ? i : 0) * pr.si;
+                rowbase[r] = rv[r] ? int64_t(i) * pr.si : 0;  // from the problem's first element
             } else if constexpr (FORM == F_SERIES) {
                 const float *xs = (const float *)a.codes_s;
 #pragma unroll
                 for (int d = 0; d < SDD; ++d) sa[r][d] = rv[r] ? xs[(pr.off_s + i) * SD + d] : 0.0f;
             }
-            Dl[r] = rv[r] ? a.border_s[i + 1] : W;  // D(i, -1)
-            Ql[r] = W;                              // Q(i, -1) (solver.h:2926-2933)
+            DlA[r] = rv[r] ? a.border_s[i + 1] : W;  // D(i, -1)
+            DlB[r] = DlA[r];
+            Ql[r] = W;                               // Q(i, -1) (solver.h:2926-2933)
         }
-        float botD = Dl[R - 1], botP = W;  // this lane's last row, handed down
+        float botD = DlA[R - 1], botP = W;  // this lane's last row, handed down
         // diagonal predecessor of the strip's first row at j = 0: D(row0-1, -1)
         float prevD = W;
         if (l == 0) prevD = (row0 == 0) ? a.border_t[0] : (act ? a.border_s[row0] : W);
 
-        // A step's global inputs are loaded one step ahead, so their latency
-        // overlaps the previous step's cells: the column's symbol (series
-        // row, or the similarity column for dense input) and, for lane 0,
-        // the row above the strip (border or row buffer).
-        int nx_sym = 0;
-        float nx_v[SDD];
-        float nx_S[FORM == F_DENSE ? R : 1];
-        float nx_D = W, nx_P = W;
-        auto fetch = [&](int s) {
+        // The column's inputs (symbol, series row or similarity column) and,
+        // for lane 0, the row above the strip (border or row buffer) are
+        // loaded one step ahead, read from these bases at the clamped column.
+        [[maybe_unused]] const uint8_t *tb8 =
+            act ? (const uint8_t *)a.codes_t + pr.off_t : (const uint8_t *)a.safe;
+        [[maybe_unused]] const uint32_t *tb32 =
+            act ? (const uint32_t *)a.codes_t + pr.off_t : (const uint32_t *)a.safe;
+        [[maybe_unused]] const float *tbf = act ? (const float *)a.codes_t + pr.off_t * SDD : a.safe;
+        [[maybe_unused]] const float *dbase = act ? a.dense + pr.off_s : safeW;
+        [[maybe_unused]] const int64_t dsj = act ? int64_t(pr.sj) : 0;
+        const float *upD = safeW, *upP = safeW;  // D(-1, j) / P(-1, j) or the strip above's bottom row
+        int upDs = 0, upPs = 0;                  // their strides in floats
+        if (l == 0 && act) {
+            if (st == 0) {
+                upD = a.border_t + 1;
+                upDs = 1;
+            } else {
+                upD = reinterpret_cast<const float *>(a.rowbuf + rb_in);
+                upP = upD + 1;
+                upDs = upPs = 2;
+            }
+        }
+        // a column's inputs, loaded two steps ahead into one of two slots
+        struct Col {
+            int sym;
+            float v[SDD];
+            float S[FORM == F_DENSE ? R : 1];
+            float D, P;
+        };
+        Col cA, cB;
+        auto fetch = [&](int s, Col &c) {
             const int j = s - l;
-            const bool on = act && j >= 0 && j < lt;
+            const int jc = act ? (j < 0 ? 0 : (j < lt ? j : lt - 1)) : 0;
             if constexpr (FORM == F_TABLE) {
-                nx_sym = on ? int(((const uint8_t *)a.codes_t)[pr.off_t + j]) : 0;
+                c.sym = int(tb8[jc]);
             } else if constexpr (FORM == F_BINARY) {
-                nx_sym = on ? int(((const uint32_t *)a.codes_t)[pr.off_t + j]) : 0;
+                c.sym = int(tb32[jc]);
             } else if constexpr (FORM == F_DENSE) {
 #pragma unroll
-                for (int r = 0; r < R; ++r)
-                    nx_S[r] = (on && rv[r]) ? a.dense[rowbase[r] + int64_t(j) * pr.sj] : W;
+                for (int r = 0; r < R; ++r) c.S[r] = dbase[rowbase[r] + int64_t(jc) * dsj];
             } else if constexpr (FORM == F_SERIES) {
-                const float *xt = (const float *)a.codes_t;
 #pragma unroll
-                for (int d = 0; d < SDD; ++d) nx_v[d] = on ? xt[(pr.off_t + j) * SD + d] : 0.0f;
+                for (int d = 0; d < SDD; ++d) c.v[d] = tbf[jc * SD + d];
             }
-            if (l == 0) {
-                nx_D = W;
-                nx_P = W;  // P(-1, j)
-                if (act && j < lt) {
-                    if (st == 0) {
-                        nx_D = a.border_t[j + 1];  // D(-1, j)
-                    } else {
-                        if constexpr (BAND) band_wait(a, pr.band + st - 1, uint32_t(j + 1), seen);
-                        const float2 v = a.rowbuf[rb_in + j];
-                        nx_D = v.x;
-                        nx_P = v.y;
-                    }
-                }
+            if constexpr (BAND) {
+                if (l == 0 && act && st > 0 && j < lt) band_wait(a, pr.band + st - 1, uint32_t(j + 1), seen);
+            }
+            c.D = upD[jc * upDs];
+            c.P = upP[jc * upPs];
+        };
+
+        // Stores of a step (the traceback words and the last lane's bottom
+        // row; in band mode also its published column count) go out at the
+        // top of the next step, after that step's loads.  Like the loads they
+        // are unconditional (a lane with nothing to store stores to a.sink),
+        // so the compiler counts every memory operation exactly and waits
+        // for a load only where its value is used, two steps after it left.
+        uint32_t *const sink_tb = a.sink;
+        float2 *const sink_rb = reinterpret_cast<float2 *>(a.sink + 8);
+        [[maybe_unused]] uint32_t *pend_dst = sink_tb;
+        [[maybe_unused]] uint32_t pend_w[WPC];
+#pragma unroll
+        for (int w = 0; w < WPC; ++w) pend_w[w] = 0;
+        float2 *pend_rb = sink_rb;
+        float2 pend_v = make_float2(0.0f, 0.0f);
+        [[maybe_unused]] bool pend_pub = false;
+        [[maybe_unused]] uint32_t pend_cols = 0;
+        auto flush = [&]() {
+            if constexpr (TR != TR_SCORE) {
+#pragma unroll
+                for (int w = 0; w < WPC; ++w) pend_dst[w] = pend_w[w];
+                pend_dst = sink_tb;
+            }
+            *pend_rb = pend_v;
+            pend_rb = sink_rb;
+            if constexpr (BAND) {
+                if (pend_pub)
+                    __hip_atomic_store(a.progress + pr.band + st, pend_cols, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                pend_pub = false;
             }
         };
-        fetch(0);
+        fetch(0, cA);
+        fetch(1, cB);
 
-        for (int s = 0; s < steps; ++s) {
+        // one step: column j = s - l of this lane's R rows; the left column
+        // D(i, j-1) is read from Din and this column written to Dout (two
+        // banks used alternately: row r+1 still needs row r's old value as
+        // its diagonal, so one bank would cost a register copy per cell)
+        auto step = [&](int s, float (&Din)[R], float (&Dout)[R], Col &cur) {
             const int j = s - l;
-            const int tsym = nx_sym;
+            const int tsym = cur.sym;
             [[maybe_unused]] float tv[SDD];
             [[maybe_unused]] float Scol[FORM == F_DENSE ? R : 1];
             if constexpr (FORM == F_SERIES) {
 #pragma unroll
-                for (int d = 0; d < SDD; ++d) tv[d] = nx_v[d];
+                for (int d = 0; d < SDD; ++d) tv[d] = cur.v[d];
             }
             if constexpr (FORM == F_DENSE) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) Scol[r] = nx_S[r];
+                for (int r = 0; r < R; ++r) Scol[r] = cur.S[r];
             }
             float inD = shr1(botD), inP = shr1(botP);
             if (l == 0) {
-                inD = nx_D;
-                inP = nx_P;
+                inD = cur.D;
+                inP = cur.P;
             }
-            fetch(s + 1);
+            fetch(s + 2, cur);
+            flush();
             const float Ddiag = prevD;
             prevD = inD;
             if (act && j >= 0 && j < lt) {
@@ -245,7 +300,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                     } else if constexpr (FORM == F_BINARY) {
                         S = rv[r] ? (rowsym[r] == tsym ? a.eq : a.ne) : W;
                     } else if constexpr (FORM == F_DENSE) {
-                        S = Scol[r];
+                        S = rv[r] ? Scol[r] : W;
                     } else {
                         // squared Euclidean cost, dimensions summed in order
                         float c = 0.0f;
@@ -264,14 +319,14 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                         float m = Dd;
                         int src = 1;
                         if (better<DIR>(Dup, m)) { m = Dup; src = 2; }
-                        if (better<DIR>(Dl[r], m)) { m = Dl[r]; src = 3; }
+                        if (better<DIR>(Din[r], m)) { m = Din[r]; src = 3; }
                         D = m + S;
                         bits = uint32_t(src);
                     } else if constexpr (GAP == GAP_LIN) {
                         // [0], diag, up, left (solver.h:2841-2852)
                         const float dg = Dd + S;
                         const float up = Dup + go_s;
-                        const float lf = Dl[r] + go_t;
+                        const float lf = Din[r] + go_t;
                         if constexpr (FAST) {
                             D = fmaxf(fmaxf(fmaxf(dg, 0.0f), up), lf);
                         } else {
@@ -294,7 +349,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                     } else {
                         // Gotoh (solver.h:2993-3033)
                         const float po = Dup + go_s, pe = Pup + ge_s;
-                        const float qo = Dl[r] + go_t, qe = Ql[r] + ge_t;
+                        const float qo = Din[r] + go_t, qe = Ql[r] + ge_t;
                         const float dg = Dd + S;
                         if constexpr (FAST) {
                             P = fmaxf(po, pe);
@@ -360,8 +415,8 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                     }
                     if constexpr (TR != TR_SCORE) tbw[(r * BITS) / 32] |= bits << ((r * BITS) % 32);
 
-                    Dd = Dl[r];
-                    Dl[r] = D;
+                    Dd = Din[r];
+                    Dout[r] = D;
                     Ql[r] = Q;
                     Dup = D;
                     Pup = P;
@@ -369,30 +424,29 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 botD = Dup;
                 botP = Pup;
                 if (l == L - 1 && st + 1 < nst) {
-                    a.rowbuf[rb_out + j] = make_float2(botD, botP);
+                    pend_rb = a.rowbuf + rb_out + j;
+                    pend_v = make_float2(botD, botP);
                     if constexpr (BAND) {
-                        if (((j + 1) % BAND_PUB) == 0 || j == lt - 1)
-                            __hip_atomic_store(a.progress + pr.band + st, uint32_t(j + 1), __ATOMIC_RELEASE,
-                                               __HIP_MEMORY_SCOPE_AGENT);
+                        pend_pub = ((j + 1) % BAND_PUB) == 0 || j == lt - 1;
+                        pend_cols = uint32_t(j + 1);
                     }
                 }
 
                 if constexpr (TR != TR_SCORE) {
                     // band-mode problems of a checkpointed forward keep no traceback
                     if (!BAND || pr.tb_off != NO_TB_OFF) {
-                        uint32_t *dst =
-                            a.tb + pr.tb_off + (int64_t(st) * nsteps_p + s) * int64_t(L) * WPC + l * WPC;
+                        pend_dst = a.tb + pr.tb_off + (int64_t(st) * nsteps_p + s) * int64_t(L) * WPC + l * WPC;
 #pragma unroll
-                        for (int w = 0; w < WPC; ++w) dst[w] = tbw[w];
+                        for (int w = 0; w < WPC; ++w) pend_w[w] = tbw[w];
                     }
                 }
 
                 // ---- seeds from the finished column (global / semiglobal) --
                 if constexpr (LOC == LOC_SEMI || LOC == LOC_GLOBAL) {
                     if (st == st_last && l == l_last) {
-                        float v = Dl[0];
+                        float v = Dout[0];
 #pragma unroll
-                        for (int r = 1; r < R; ++r) v = (r == r_last) ? Dl[r] : v;
+                        for (int r = 1; r < R; ++r) v = (r == r_last) ? Dout[r] : v;
                         if constexpr (LOC == LOC_SEMI) {
                             // last row, j ascending, strict (solver.h:2285-2287)
                             if (better<DIR>(v, rbest)) { rbest = v; rj = j; }
@@ -405,12 +459,19 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                             // last column, i ascending, strict (solver.h:2289-2291)
 #pragma unroll
                             for (int r = 0; r < R; ++r)
-                                if (rv[r] && better<DIR>(Dl[r], cbest)) { cbest = Dl[r]; ci = row0 + r; }
+                                if (rv[r] && better<DIR>(Dout[r], cbest)) { cbest = Dout[r]; ci = row0 + r; }
                         }
                     }
                 }
             }
+        };
+        int s = 0;
+        for (; s + 1 < steps; s += 2) {
+            step(s, DlA, DlB, cA);
+            step(s + 1, DlB, DlA, cB);
         }
+        if (s < steps) step(s, DlA, DlB, cA);
+        flush();
         // the row buffer written by this strip's last lane is read next strip
         __threadfence_block();
     }
