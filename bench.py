@@ -118,6 +118,34 @@ def c3_inputs(seed=3):
     return "".join(alphabet[A]), "".join(alphabet[B])
 
 
+def c1_inputs(n=4096, length=100, seed=1):
+    """4,096 pairs of uniform ACGT strings, both of length 100."""
+    rng = np.random.default_rng(seed)
+    letters = np.array(list("ACGT"))
+    return [("".join(rng.choice(letters, length)), "".join(rng.choice(letters, length))) for _ in range(n)]
+
+
+def c1_bench(reps):
+    """Plain global alignment through the public API: GlobalSolver(LinearGapCost(0.2),
+    codomain=Solution) on 4,096 binary(eq=1, ne=-1) problems (problem objects made
+    before the timed region; each solve returns 4,096 Solutions with their paths)."""
+    import pyalign_amd as pa
+    pairs = c1_inputs()
+    pf = pa.binary(eq=1, ne=-1)
+    probs = [pf.new_problem(a, b) for a, b in pairs]
+    solver = pa.GlobalSolver(gap_cost=pa.LinearGapCost(0.2), codomain=pa.Solution)
+    solver.solve(probs)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sols = solver.solve(probs)
+    secs = (time.perf_counter() - t0) / reps
+    cells = float(len(pairs) * 100 * 100)
+    return {"workload": "plain global alignment, binary(eq=1, ne=-1), LinearGapCost(0.2): 4,096 pairs of "
+                        "100 x 100 ACGT through GlobalSolver(codomain=Solution).solve",
+            "solve_s": round(secs, 4), "gcups": round(cells / secs / 1e9, 3),
+            "first_score": float(sols[0].score)}
+
+
 def c3_bench(reps):
     """Single long semiglobal affine pair, score-only and single Solution."""
     import pyalign_amd as pa
@@ -257,7 +285,7 @@ def make_solver(lim_s, lim_t):
     return algorithm.m16.create_solver(lim_s, lim_t, opts)
 
 
-def cpu_baseline(table, ls, lt, codes_s, codes_t, off_s, off_t, sample, lim):
+def cpu_baseline(table, ls, lt, codes_s, codes_t, off_s, off_t, sample, lim, threads=None):
     """The oracle's 16-lane batched score path (reference batching) on the host."""
     import oracle
     import pyalign_amd as pa
@@ -276,7 +304,8 @@ def cpu_baseline(table, ls, lt, codes_s, codes_t, off_s, off_t, sample, lim):
     for b in bag.batches(16):  # instance.py:230-275 with 16 AVX-512 lanes
         members += list(b.indices)
         boff.append(len(members))
-    threads = min(16, os.cpu_count() or 1)
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
     cfg = oracle.make_config(locality=oracle.LOCAL, gap=("affine", 10, 1), goal=oracle.GOAL_SCORE,
                              lim_s=lim, lim_t=lim)
     secs, scores = oracle.baseline_indexed_score(
@@ -358,6 +387,21 @@ def main():
     achieved = cells * OPS_PER_CELL / (kms * 1e-3) / 1e12
 
     log(f"batched: {ms_per_step:.2f} ms/step, {value:.1f} GCUPS")
+    # the same solve from host buffers: codes uploaded and scores read back per call
+    host = None
+    if not args.no_extra:
+        solver.solve_ragged("score", "indexed", ls, lt, codes_s=codes_s, codes_t=codes_t, off_s=off_s, off_t=off_t,
+                            table=table)
+        th0 = time.perf_counter()
+        for _ in range(3):
+            hs = solver.solve_ragged("score", "indexed", ls, lt, codes_s=codes_s, codes_t=codes_t, off_s=off_s,
+                                     off_t=off_t, table=table)
+        hms = (time.perf_counter() - th0) * 1e3 / 3
+        hs = np.asarray(hs, dtype=np.float32)
+        host = {"workload": "the same bag from host buffers (69 MB of codes uploaded, scores read back, per call)",
+                "ms_per_solve": round(hms, 3), "gcups": round(cells / (hms * 1e-3) / 1e9, 2),
+                "scores_bitwise_equal": bool(np.array_equal(hs.view(np.uint32), auto_scores.view(np.uint32)))}
+        log(f"host buffers: {hms:.2f} ms per solve")
     plain = None
     if not args.no_plain:
         step(_lib.PA_SCHED_PLAIN)
@@ -380,15 +424,19 @@ def main():
     if not args.no_cpu:
         sample = min(args.cpu_sample, n)
         gcups, threads, secs, cpu_scores = cpu_baseline(table, ls, lt, codes_s, codes_t, off_s, off_t, sample, lim)
+        g1, _, secs1, _ = cpu_baseline(table, ls, lt, codes_s, codes_t, off_s, off_t, min(2048, n), lim, threads=1)
         cpu = {"value": round(gcups, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+               "single_core": {"value": round(g1, 4), "sample": f"first {min(2048, n)} pairs ({secs1:.1f} s)"},
                "sample": (f"all {n} pairs" if sample == n else f"first {sample} of the {n} pairs")
                          + f" ({secs:.1f} s; oracle's 16-lane batched restatement, Morton batches of 16 as the "
                            "reference's ProblemBag.batches)",
                "scores_bitwise_equal": bool(np.array_equal(cpu_scores.view(np.uint32),
                                                            auto_scores[:sample].view(np.uint32)))}
 
-    long_pair = fwd_bwd = dtw = api = None
+    long_pair = fwd_bwd = dtw = api = c1 = None
     if not args.no_extra:
+        c1 = c1_bench(2)
+        log(f"global Solution batch: {c1['solve_s']:.3f} s per solve")
         api, api_scores = api_bench(table, ls, lt, codes_s, codes_t)
         api["scores_bitwise_equal"] = bool(np.array_equal(api_scores.view(np.uint32), auto_scores.view(np.uint32)))
         log(f"public API: {api['solve_s']:.3f} s per solve, {api['gcups']:.1f} GCUPS")
@@ -416,7 +464,9 @@ def main():
         "fwd_bwd": fwd_bwd,
         "long_pair": long_pair,
         "dtw_fwd_bwd": dtw,
+        "host_buffers": host,
         "public_api": api,
+        "global_solution_batch": c1,
     }
     print(json.dumps(out))
 

@@ -663,6 +663,8 @@ struct Input {
     const uint8_t *lut = nullptr;  // indexed form: symbol -> code, 255 = none
     int32_t lut_len = 0;
     int64_t syms_s = 0, syms_t = 0;  // symbols per side
+    std::vector<uint8_t> lut_store;  // an identity table that checks host codes on the device
+    const char *bad_s = "item not in alphabet", *bad_t = "item not in alphabet";
 };
 
 int goal_to_tr(int goal) { return goal == PA_GOAL_SCORE ? TR_SCORE : (goal == PA_GOAL_ONE ? TR_ONE : TR_ALL); }
@@ -725,7 +727,8 @@ void stage_payload(pa_solver &s, const Input &in, hipStream_t st, const void *&c
         HIPCHK(hipMemcpyAsync(first, bad, sizeof(first), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         for (int side = 0; side < 2; side++) s.last_bad[side] = first[side] == ~0ull ? -1 : int64_t(first[side]);
-        if (s.last_bad[0] >= 0 || s.last_bad[1] >= 0) fail(PA_EINVAL, "item not in alphabet");
+        if (s.last_bad[0] >= 0) fail(PA_EINVAL, in.bad_s);
+        if (s.last_bad[1] >= 0) fail(PA_EINVAL, in.bad_t);
     }
 }
 
@@ -1588,15 +1591,18 @@ static void ragged_input(const pa_solver *s, const pa_ragged *b, Input &in) {
                 in.lut = b->lut;
                 in.lut_len = b->lut_len;
             } else if (!b->on_device) {
-                // every code indexes the table staged in LDS
-                const uint8_t *cs = static_cast<const uint8_t *>(b->codes_s);
-                const uint8_t *ct = static_cast<const uint8_t *>(b->codes_t);
-                for (int64_t k = 0; k < b->n; k++) {
-                    for (int64_t i = 0; i < b->len_s[k]; i++)
-                        if (cs[b->off_s[k] + i] >= b->K) fail(PA_EINVAL, "out of bounds index in a");
-                    for (int64_t j = 0; j < b->len_t[k]; j++)
-                        if (ct[b->off_t[k] + j] >= b->K) fail(PA_EINVAL, "out of bounds index in b");
-                }
+                // every code indexes the table staged in LDS: host codes go
+                // through the device encoder with an identity table, which
+                // flags codes >= K before any DP launch
+                in.lut_store.assign(256, 255);
+                for (int32_t c = 0; c < b->K && c < 255; c++) in.lut_store[size_t(c)] = uint8_t(c);
+                in.lut = in.lut_store.data();
+                in.lut_len = 256;
+                in.raw_width = 1;
+                in.syms_s = end_s;
+                in.syms_t = end_t;
+                in.bad_s = "out of bounds index in a";
+                in.bad_t = "out of bounds index in b";
             }
             bool nan_free = true;
             in.table = make_table(b->table, b->K, b->K, s->opt.dir, &nan_free);

@@ -124,3 +124,37 @@ def test_symbol_outside_alphabet_errors():
     # the solver stays usable after the error
     assert solver.solve([pf.new_problem("ACD", "ACD")])[0] == pytest.approx(
         solver.solve([pf.new_problem(list("ACD"), list("ACD"))])[0])
+
+
+def test_ragged_codes_beyond_the_table_raise():
+    """Host uint8 codes of the ragged indexed form are checked on the device
+    (an identity code table) before any DP work: a code >= K raises the
+    reference's error for its side; valid codes solve as before."""
+    from pyalign_amd import algorithm
+    rng = np.random.default_rng(46)
+    opts = algorithm.m16.create_options(dict(
+        solver=algorithm.Type.ALIGNMENT, batch=True, direction=algorithm.Direction.MAXIMIZE, return_dup=False,
+        codomain=pa.solve.Codomain(pa.Score), locality=algorithm.Locality.LOCAL, gap_cost=pa.AffineGapCost(3, 1)))
+    solver = algorithm.m16.create_solver(64, 64, opts)
+    ls = np.array([10, 20, 5], np.int32)
+    lt = np.array([7, 30, 9], np.int32)
+    cs = rng.integers(0, 4, int(ls.sum())).astype(np.uint8)
+    ct = rng.integers(0, 4, int(lt.sum())).astype(np.uint8)
+    os_ = np.array([0, 10, 30], np.int64)
+    ot = np.array([0, 7, 37], np.int64)
+    table = np.eye(4, dtype=np.float32) * 3 - 1
+    ok = solver.solve_ragged("score", "indexed", ls, lt, codes_s=cs, codes_t=ct, off_s=os_, off_t=ot, table=table)
+    for k in range(3):
+        S = table[cs[os_[k]:os_[k] + ls[k]]][:, ct[ot[k]:ot[k] + lt[k]]]
+        ref = O.solve(S, locality=O.LOCAL, gap=("affine", 3, 1), lim_s=64, lim_t=64)
+        assert np.float32(ok[k]).tobytes() == ref.score.tobytes()
+    bad = cs.copy()
+    bad[12] = 4
+    with pytest.raises(ValueError, match="out of bounds index in a"):
+        solver.solve_ragged("score", "indexed", ls, lt, codes_s=bad, codes_t=ct, off_s=os_, off_t=ot, table=table)
+    bad = ct.copy()
+    bad[40] = 200
+    with pytest.raises(ValueError, match="out of bounds index in b"):
+        solver.solve_ragged("score", "indexed", ls, lt, codes_s=cs, codes_t=bad, off_s=os_, off_t=ot, table=table)
+    again = solver.solve_ragged("score", "indexed", ls, lt, codes_s=cs, codes_t=ct, off_s=os_, off_t=ot, table=table)
+    assert np.array_equal(np.asarray(again, np.float32).view(np.uint32), np.asarray(ok, np.float32).view(np.uint32))
