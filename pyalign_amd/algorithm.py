@@ -136,12 +136,13 @@ class Options:
         else:
             self.detail, self.count = Detail.SOLUTION, Count.ONE
         h = ctypes.c_void_p()
+        self._free = lib().pa_options_free  # held so that teardown at exit needs no module global
         check(lib().pa_create_options(ctypes.byref(desc), ctypes.byref(h)))
         self._h = h
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().pa_options_free(self._h)
+            self._free(self._h)
             self._h = None
 
     def to_dict(self):
@@ -243,6 +244,7 @@ class _Result:
     """Owns one pa_result handle."""
 
     def __init__(self, h, n, index_dtype, len_s, len_t):
+        self._free = lib().pa_result_free
         self.h = h
         self.n = n
         self.index_dtype = index_dtype
@@ -251,7 +253,7 @@ class _Result:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().pa_result_free(self.h)
+            self._free(self.h)
             self.h = None
 
     def scores(self):
@@ -331,6 +333,7 @@ class Solver:
     """Solver<Value, Index> (factory.h:414-508) on the MI355X."""
 
     def __init__(self, h, opts, max_len_s, max_len_t):
+        self._free = lib().pa_solver_free
         self._h = h
         self._opts = opts
         self._index_dtype = np.int16 if opts.index_bits == 16 else np.int32
@@ -338,7 +341,7 @@ class Solver:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().pa_solver_free(self._h)
+            self._free(self._h)
             self._h = None
 
     @property

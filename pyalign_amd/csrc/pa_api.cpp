@@ -185,6 +185,8 @@ struct pa_solver {
     bool use_graphs = true;
     // fast kernel: single-symbol LDS table even where the pair table fits
     bool no_pair = std::getenv("PYALIGN_AMD_NO_PAIR_TABLE") != nullptr;
+    // dense input: per-cell loads even where rows could be staged in LDS
+    bool no_drows = std::getenv("PYALIGN_AMD_NO_DENSE_STAGING") != nullptr;
     // a tape's global pairs longer than this (max of the two lengths) keep
     // only strip boundary rows and are recomputed strip by strip in the
     // backward; negative: never
@@ -1072,6 +1074,11 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     // ---- launches: specs first (occupancy queries stay outside any capture)
     std::vector<LaunchSpec> specs;
     std::vector<KArgs> largs;
+    // dense rows with unit column stride and >= 8 columns in every non-empty
+    // problem: the kernel stages each lane's rows through LDS (F_DROWS)
+    bool drows = job.form == F_DENSE && (in.dense_sj_fixed == 0 || in.dense_sj_fixed == 1) && !s.no_drows;
+    for (int64_t k = 0; k < n && drows; k++)
+        if (in.len_s[k] > 0 && in.len_t[k] > 0 && in.len_t[k] < DROWS_MIN_LT) drows = false;
     for (size_t li = 0; li < plan.launches.size(); li++) {
         int R = plan.launches[li].first;
         bool fastmax = job.fast;
@@ -1091,6 +1098,10 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         spec.R = R;
         spec.fastmax = fastmax;
         spec.lds_bytes = job.form == F_TABLE ? size_t(in.K1) * in.K1 * 4 : 0;
+        if (drows && R == DROWS_WIN) {
+            spec.form = F_DROWS;
+            spec.lds_bytes = DROWS_LDS;
+        }
         if (fastmax && R == 0 && in.K1 >= 2) {
             const size_t pair_bytes = size_t(in.K1) * in.K1 * (in.K1 - 1) * 8;
             if (pair_bytes <= PAIR_LDS_MAX && !s.no_pair) {

@@ -12,7 +12,15 @@ enum { GAP_LIN = 0, GAP_AFF = 1, GAP_DTW = 2, GAP_GEN = 3 };
 // Similarity sources.  F_TABLE reads a (K+1)x(K+1) table (row = s symbol,
 // last symbol = pad whose entries are the worst value) staged in LDS with
 // uint8 codes; F_TABLE_G reads it from global memory with uint16 codes.
-enum { F_DENSE = 0, F_TABLE = 1, F_BINARY = 2, F_SERIES = 3, F_TABLE_G = 4 };
+// F_DROWS is F_DENSE with every problem's rows contiguous (column stride 1)
+// and at least DROWS_MIN_LT columns: each lane stages windows of its rows
+// through LDS with 16-byte loads (a launch-level choice; the payload and
+// everything outside the DP kernel treat it as F_DENSE).
+enum { F_DENSE = 0, F_TABLE = 1, F_BINARY = 2, F_SERIES = 3, F_TABLE_G = 4, F_DROWS = 5 };
+constexpr int DROWS_WIN = 8;       // columns per staged window (== the strip height R)
+constexpr int DROWS_MIN_LT = 8;
+constexpr int DROWS_LANE_FLOATS = 2 * DROWS_WIN * 8 + 16;  // two windows of R = 8 rows, 8 floats of slack each side
+constexpr size_t DROWS_LDS = size_t(256) * DROWS_LANE_FLOATS * 4;
 // What the forward pass keeps: score only, one traceback direction per cell
 // (2 bits linear/DTW, 4 bits affine) or the full tie set (8 bits per cell).
 enum { TR_SCORE = 0, TR_ONE = 1, TR_ALL = 2 };

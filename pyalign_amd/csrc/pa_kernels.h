@@ -86,10 +86,15 @@ __device__ __forceinline__ void band_wait(const KArgs &a, int idx, uint32_t need
     }
 }
 
+extern __shared__ float pa_dyn_lds[];  // the launch's dynamic LDS (table or staging windows)
+
 template <int LOC, int DIR, int GAP, int FORM, int TR, int R, bool FAST, int SD, bool BAND = false>
 __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, const float *__restrict__ tab,
                                             int lane) {
     constexpr float W = worst<DIR>();
+    constexpr bool DENSE = FORM == F_DENSE || FORM == F_DROWS;
+    constexpr bool STAGED = FORM == F_DROWS;
+    static_assert(!STAGED || R == DROWS_WIN, "staged dense rows need R == 8");
     constexpr int BITS = TR == TR_ALL ? 8 : (TR == TR_ONE ? (GAP == GAP_AFF ? 4 : 2) : 0);
     constexpr int WPC = TR == TR_SCORE ? 1 : (BITS * R + 31) / 32;
     constexpr int SDD = SD > 0 ? SD : 1;
@@ -158,7 +163,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
             } else if constexpr (FORM == F_BINARY) {
                 const uint32_t *cs = (const uint32_t *)a.codes_s;
                 rowsym[r] = rv[r] ? int(cs[pr.off_s + i]) : 0;
-            } else if constexpr (FORM == F_DENSE) {
+            } else if constexpr (DENSE) {
                 rowbase[r] = rv[r] ? int64_t(i) * pr.si : 0;  // from the problem's first element
             } else if constexpr (FORM == F_SERIES) {
                 const float *xs = (const float *)a.codes_s;
@@ -200,7 +205,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         struct Col {
             int sym;
             float v[SDD];
-            float S[FORM == F_DENSE ? R : 1];
+            float S[DENSE ? R : 1];
             float D, P;
         };
         Col cA, cB;
@@ -259,15 +264,51 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         fetch(0, cA);
         fetch(1, cB);
 
+        // F_DROWS: window w holds this lane's R rows at columns [8w - l,
+        // 8w - l + 8), read from global memory with two 16-byte loads per row
+        // at a start clamped into the problem, kept in LDS (two windows per
+        // lane, used alternately).  Row q of window w+1 is loaded four steps
+        // before it is written to LDS at step 8w + q, during window w, so the
+        // loads have four steps to arrive; step s reads its column from
+        // window s / 8 with the window's offset (start - clamped start).
+        [[maybe_unused]] float *lw = STAGED ? pa_dyn_lds + threadIdx.x * DROWS_LANE_FLOATS + 8 : nullptr;
+        [[maybe_unused]] float stg[4][DROWS_WIN];
+        auto wclamp = [&](int w) { return act ? min(max(DROWS_WIN * w - l, 0), lt - DROWS_WIN) : 0; };
+        auto wload = [&](int w, int q, float (&dst)[DROWS_WIN]) {
+            const float *src = dbase + rowbase[q] + wclamp(w);
+            float4 v0, v1;
+            __builtin_memcpy(&v0, src, 16);
+            __builtin_memcpy(&v1, src + 4, 16);
+            dst[0] = v0.x; dst[1] = v0.y; dst[2] = v0.z; dst[3] = v0.w;
+            dst[4] = v1.x; dst[5] = v1.y; dst[6] = v1.z; dst[7] = v1.w;
+        };
+        auto wstore = [&](int w, int q, const float (&src)[DROWS_WIN]) {
+            float *d = lw + (w & 1) * (R * DROWS_WIN) + q * DROWS_WIN;
+            *reinterpret_cast<float4 *>(d) = make_float4(src[0], src[1], src[2], src[3]);
+            *reinterpret_cast<float4 *>(d + 4) = make_float4(src[4], src[5], src[6], src[7]);
+        };
+        [[maybe_unused]] const float *wread = lw;  // window of the current step, shifted by its offset
+        if constexpr (STAGED) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {  // window 0, then rows 0..3 of window 1 in flight
+                float tmp[DROWS_WIN];
+                wload(0, q, tmp);
+                wstore(0, q, tmp);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) wload(1, q, stg[q]);
+        }
+
         // one step: column j = s - l of this lane's R rows; the left column
         // D(i, j-1) is read from Din and this column written to Dout (two
         // banks used alternately: row r+1 still needs row r's old value as
         // its diagonal, so one bank would cost a register copy per cell)
-        auto step = [&](int s, float (&Din)[R], float (&Dout)[R], Col &cur) {
+        auto step = [&](auto phase, int s, float (&Din)[R], float (&Dout)[R], Col &cur) {
+            constexpr int C = decltype(phase)::value;  // s % 8 (staged dense rows only)
             const int j = s - l;
             const int tsym = cur.sym;
             [[maybe_unused]] float tv[SDD];
-            [[maybe_unused]] float Scol[FORM == F_DENSE ? R : 1];
+            [[maybe_unused]] float Scol[DENSE ? R : 1];
             if constexpr (FORM == F_SERIES) {
 #pragma unroll
                 for (int d = 0; d < SDD; ++d) tv[d] = cur.v[d];
@@ -275,6 +316,20 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
             if constexpr (FORM == F_DENSE) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) Scol[r] = cur.S[r];
+            }
+            if constexpr (STAGED) {
+                const int w = s >> 3;
+                wstore(w + 1, C, stg[C & 3]);      // row C of the next window, loaded four steps ago
+                wload(w + ((C + 4) >> 3) + 1, (C + 4) & 7, stg[C & 3]);
+                if constexpr (C == 0) {
+                    // the window's offset; within +-7 wherever a cell of this
+                    // lane is computed, clamped elsewhere so the address stays
+                    // inside the lane's slack
+                    const int off = act ? min(max(DROWS_WIN * w - l - wclamp(w), -8), 8) : 0;
+                    wread = lw + (w & 1) * (R * DROWS_WIN) + off;
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) Scol[r] = wread[r * DROWS_WIN + C];
             }
             float inD = shr1(botD), inP = shr1(botP);
             if (l == 0) {
@@ -299,7 +354,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                         S = tab[rowsym[r] + tsym];
                     } else if constexpr (FORM == F_BINARY) {
                         S = rv[r] ? (rowsym[r] == tsym ? a.eq : a.ne) : W;
-                    } else if constexpr (FORM == F_DENSE) {
+                    } else if constexpr (DENSE) {
                         S = rv[r] ? Scol[r] : W;
                     } else {
                         // squared Euclidean cost, dimensions summed in order
@@ -465,12 +520,27 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 }
             }
         };
+        using std::integral_constant;
         int s = 0;
-        for (; s + 1 < steps; s += 2) {
-            step(s, DlA, DlB, cA);
-            step(s + 1, DlB, DlA, cB);
+        if constexpr (STAGED) {
+            // eight steps a round: the phase s % 8 is a compile-time constant
+            for (; s < steps; s += 8) {
+                step(integral_constant<int, 0>{}, s, DlA, DlB, cA);
+                if (s + 1 < steps) step(integral_constant<int, 1>{}, s + 1, DlB, DlA, cB);
+                if (s + 2 < steps) step(integral_constant<int, 2>{}, s + 2, DlA, DlB, cA);
+                if (s + 3 < steps) step(integral_constant<int, 3>{}, s + 3, DlB, DlA, cB);
+                if (s + 4 < steps) step(integral_constant<int, 4>{}, s + 4, DlA, DlB, cA);
+                if (s + 5 < steps) step(integral_constant<int, 5>{}, s + 5, DlB, DlA, cB);
+                if (s + 6 < steps) step(integral_constant<int, 6>{}, s + 6, DlA, DlB, cA);
+                if (s + 7 < steps) step(integral_constant<int, 7>{}, s + 7, DlB, DlA, cB);
+            }
+        } else {
+            for (; s + 1 < steps; s += 2) {
+                step(integral_constant<int, 0>{}, s, DlA, DlB, cA);
+                step(integral_constant<int, 0>{}, s + 1, DlB, DlA, cB);
+            }
+            if (s < steps) step(integral_constant<int, 0>{}, s, DlA, DlB, cA);
         }
-        if (s < steps) step(s, DlA, DlB, cA);
         flush();
         // the row buffer written by this strip's last lane is read next strip
         __threadfence_block();
@@ -834,6 +904,10 @@ static hipError_t dispatch_lg(const LaunchSpec &s, const KArgs &a, hipStream_t s
             if (s.tr == TR_SCORE) PA_GO(F_DENSE, TR_SCORE, 8, false, 0);
             PA_GO(F_DENSE, TR_ONE, 8, false, 0);
         }
+        if (s.form == F_DROWS) {
+            if (s.tr == TR_SCORE) PA_GO(F_DROWS, TR_SCORE, 8, false, 0);
+            PA_GO(F_DROWS, TR_ONE, 8, false, 0);
+        }
         if (s.form == F_SERIES) {
             switch (a.dim) {
                 case 1: if (s.tr == TR_SCORE) PA_GO(F_SERIES, TR_SCORE, 8, false, 1); PA_GO(F_SERIES, TR_ONE, 8, false, 1);
@@ -892,6 +966,10 @@ static hipError_t dispatch_lg(const LaunchSpec &s, const KArgs &a, hipStream_t s
                 if (s.tr == TR_SCORE) PA_GO(F_DENSE, TR_SCORE, 8, false, 0);
                 if (s.tr == TR_ONE) PA_GO(F_DENSE, TR_ONE, 8, false, 0);
                 PA_GO(F_DENSE, TR_ALL, 8, false, 0);
+            case F_DROWS:
+                if (s.tr == TR_SCORE) PA_GO(F_DROWS, TR_SCORE, 8, false, 0);
+                if (s.tr == TR_ONE) PA_GO(F_DROWS, TR_ONE, 8, false, 0);
+                PA_GO(F_DROWS, TR_ALL, 8, false, 0);
             default: return hipErrorInvalidValue;
         }
     }

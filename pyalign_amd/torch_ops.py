@@ -35,13 +35,14 @@ _LOC = {"local": algorithm.Locality.LOCAL, "global": algorithm.Locality.GLOBAL,
 
 class _Tape:
     def __init__(self, h, solver, keep):
+        self._free = lib().pa_tape_free
         self.h = h
         self.solver = solver  # the tape borrows the solver and the payload
         self.keep = keep
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().pa_tape_free(self.h)
+            self._free(self.h)
             self.h = None
 
 
