@@ -197,14 +197,12 @@ struct pa_solver {
     int last_launches = 0;
     double last_cells = 0.0;
     int last_groups = 0, last_band = 0, last_ckpt = 0;
+    // tapes made by this solver and not yet freed: a freed tape hands its
+    // device buffers back (the next forward reuses them instead of a
+    // hipMalloc / hipFree pair per step); the solver going first detaches them
+    std::vector<pa_tape *> tapes;
 
-    ~pa_solver() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (band_err_host) (void)hipHostFree(band_err_host);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
+    ~pa_solver();
 };
 
 namespace {
@@ -269,6 +267,15 @@ struct pa_tape {
     // backward scratch
     DevBuf s_probs, s_groups, s_queue, s_tb, s_progress, s_partials, s_err, s_state, s_strip;
 };
+
+pa_solver::~pa_solver() {
+    for (pa_tape *t : tapes) t->solver = nullptr;
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (band_err_host) (void)hipHostFree(band_err_host);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
 
 namespace {
 
@@ -1074,8 +1081,8 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     // ---- launches: specs first (occupancy queries stay outside any capture)
     std::vector<LaunchSpec> specs;
     std::vector<KArgs> largs;
-    // dense rows with unit column stride and >= 8 columns in every non-empty
-    // problem: the kernel stages each lane's rows through LDS (F_DROWS)
+    // dense rows with unit column stride and >= DROWS_MIN_LT columns in every
+    // non-empty problem: the kernel stages each lane's rows through LDS (F_DROWS)
     bool drows = job.form == F_DENSE && (in.dense_sj_fixed == 0 || in.dense_sj_fixed == 1) && !s.no_drows;
     for (int64_t k = 0; k < n && drows; k++)
         if (in.len_s[k] > 0 && in.len_t[k] > 0 && in.len_t[k] < DROWS_MIN_LT) drows = false;
@@ -1202,6 +1209,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
 
     if (tape) {
         tape->solver = &s;
+        s.tapes.push_back(tape);
         tape->n = n;
         tape->n_probs = int64_t(plan.probs.size());
         tape->loc = job.loc;
@@ -1920,7 +1928,8 @@ extern "C" {
 int pa_backward(pa_tape *t, const float *grad_scores_device, float *grad_dense_device,
                 float *grad_series_s_device, float *grad_series_t_device, float *grad_gap_device) {
     return guard([&] {
-        if (!t || !t->solver || !grad_scores_device) fail(PA_EINVAL, "null argument");
+        if (!t || !grad_scores_device) fail(PA_EINVAL, "null argument");
+        if (!t->solver) fail(PA_EINVAL, "the tape's solver was freed");
         BwArgs a{};
         a.probs = t->probs.as<Prob>();
         a.n = t->n_probs;
@@ -1944,7 +1953,27 @@ int pa_backward(pa_tape *t, const float *grad_scores_device, float *grad_dense_d
     });
 }
 
-void pa_tape_free(pa_tape *t) { delete t; }
+void pa_tape_free(pa_tape *t) {
+    if (!t) return;
+    if (pa_solver *s = t->solver) {
+        s->tapes.erase(std::remove(s->tapes.begin(), s->tapes.end(), t), s->tapes.end());
+        // the larger of each pair stays with the solver (the tape's work is
+        // done: pa_backward synchronises, and the solver's later work is
+        // ordered after it on the same stream)
+        auto give_back = [](DevBuf &from, DevBuf &to) {
+            if (from.cap <= to.cap) return false;
+            std::swap(from.p, to.p);
+            std::swap(from.cap, to.cap);
+            return true;
+        };
+        // the cached plan's descriptors live in b_probs
+        if (give_back(t->probs, s->b_probs)) s->last_plan.valid = false;
+        give_back(t->seeds, s->b_seeds);
+        give_back(t->tb, s->b_tb);
+        give_back(t->rowbuf, s->b_rowbuf);
+    }
+    delete t;
+}
 
 // ---- results ---------------------------------------------------------------
 

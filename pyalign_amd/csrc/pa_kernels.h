@@ -303,8 +303,9 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         // D(i, j-1) is read from Din and this column written to Dout (two
         // banks used alternately: row r+1 still needs row r's old value as
         // its diagonal, so one bank would cost a register copy per cell)
-        auto step = [&](auto phase, int s, float (&Din)[R], float (&Dout)[R], Col &cur) {
+        auto step = [&](auto phase, auto keepv, int s, float (&Din)[R], float (&Dout)[R], Col &cur) {
             constexpr int C = decltype(phase)::value;  // s % 8 (staged dense rows only)
+            constexpr bool KEEPV = decltype(keepv)::value;  // cell values stored (a.dvals set)
             const int j = s - l;
             const int tsym = cur.sym;
             [[maybe_unused]] float tv[SDD];
@@ -455,8 +456,8 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                     }
                     // cell values: local all-optimal seeds read D; the Solution
                     // matrices read every layer (solver.h:2355-2375)
-                    if constexpr (TR != TR_SCORE) {
-                        if (a.dvals != nullptr && rv[r]) {
+                    if constexpr (TR != TR_SCORE && KEEPV) {
+                        if (rv[r]) {
                             float *dv = a.dvals + pr.dv_off + int64_t(i) * lt + j;
                             dv[0] = D;
                             if constexpr (GAP == GAP_AFF) {
@@ -497,49 +498,65 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 }
 
                 // ---- seeds from the finished column (global / semiglobal) --
-                if constexpr (LOC == LOC_SEMI || LOC == LOC_GLOBAL) {
+                // (global: D(ls-1, lt-1) is read once the strip is done)
+                if constexpr (LOC == LOC_SEMI) {
                     if (st == st_last && l == l_last) {
                         float v = Dout[0];
 #pragma unroll
                         for (int r = 1; r < R; ++r) v = (r == r_last) ? Dout[r] : v;
-                        if constexpr (LOC == LOC_SEMI) {
-                            // last row, j ascending, strict (solver.h:2285-2287)
-                            if (better<DIR>(v, rbest)) { rbest = v; rj = j; }
-                        } else {
-                            if (j == lt - 1) gscore = v;
-                        }
+                        // last row, j ascending, strict (solver.h:2285-2287)
+                        if (better<DIR>(v, rbest)) { rbest = v; rj = j; }
                     }
-                    if constexpr (LOC == LOC_SEMI) {
-                        if (j == lt - 1) {
-                            // last column, i ascending, strict (solver.h:2289-2291)
+                    if (j == lt - 1) {
+                        // last column, i ascending, strict (solver.h:2289-2291)
 #pragma unroll
-                            for (int r = 0; r < R; ++r)
-                                if (rv[r] && better<DIR>(Dout[r], cbest)) { cbest = Dout[r]; ci = row0 + r; }
-                        }
+                        for (int r = 0; r < R; ++r)
+                            if (rv[r] && better<DIR>(Dout[r], cbest)) { cbest = Dout[r]; ci = row0 + r; }
                     }
                 }
             }
         };
         using std::integral_constant;
-        int s = 0;
-        if constexpr (STAGED) {
-            // eight steps a round: the phase s % 8 is a compile-time constant
-            for (; s < steps; s += 8) {
-                step(integral_constant<int, 0>{}, s, DlA, DlB, cA);
-                if (s + 1 < steps) step(integral_constant<int, 1>{}, s + 1, DlB, DlA, cB);
-                if (s + 2 < steps) step(integral_constant<int, 2>{}, s + 2, DlA, DlB, cA);
-                if (s + 3 < steps) step(integral_constant<int, 3>{}, s + 3, DlB, DlA, cB);
-                if (s + 4 < steps) step(integral_constant<int, 4>{}, s + 4, DlA, DlB, cA);
-                if (s + 5 < steps) step(integral_constant<int, 5>{}, s + 5, DlB, DlA, cB);
-                if (s + 6 < steps) step(integral_constant<int, 6>{}, s + 6, DlA, DlB, cA);
-                if (s + 7 < steps) step(integral_constant<int, 7>{}, s + 7, DlB, DlA, cB);
+        // the step loop, built twice where cell values may be kept so that
+        // the usual loop carries no per-cell test of a.dvals
+        auto sweep = [&](auto kv) {
+            int s = 0;
+            if constexpr (STAGED) {
+                // eight steps a round: the phase s % 8 is a compile-time constant
+                for (; s < steps; s += 8) {
+                    step(integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
+                    if (s + 1 < steps) step(integral_constant<int, 1>{}, kv, s + 1, DlB, DlA, cB);
+                    if (s + 2 < steps) step(integral_constant<int, 2>{}, kv, s + 2, DlA, DlB, cA);
+                    if (s + 3 < steps) step(integral_constant<int, 3>{}, kv, s + 3, DlB, DlA, cB);
+                    if (s + 4 < steps) step(integral_constant<int, 4>{}, kv, s + 4, DlA, DlB, cA);
+                    if (s + 5 < steps) step(integral_constant<int, 5>{}, kv, s + 5, DlB, DlA, cB);
+                    if (s + 6 < steps) step(integral_constant<int, 6>{}, kv, s + 6, DlA, DlB, cA);
+                    if (s + 7 < steps) step(integral_constant<int, 7>{}, kv, s + 7, DlB, DlA, cB);
+                }
+            } else {
+                for (; s + 1 < steps; s += 2) {
+                    step(integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
+                    step(integral_constant<int, 0>{}, kv, s + 1, DlB, DlA, cB);
+                }
+                if (s < steps) step(integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
             }
+        };
+        if constexpr (TR != TR_SCORE) {
+            if (a.dvals != nullptr) sweep(std::true_type{});
+            else sweep(std::false_type{});
         } else {
-            for (; s + 1 < steps; s += 2) {
-                step(integral_constant<int, 0>{}, s, DlA, DlB, cA);
-                step(integral_constant<int, 0>{}, s + 1, DlB, DlA, cB);
+            sweep(std::false_type{});
+        }
+        if constexpr (LOC == LOC_GLOBAL) {
+            // D(ls-1, lt-1): column lt-1 was this lane's step lt-1+l, which
+            // wrote DlB when even and DlA when odd; later steps write nothing
+            if (st == st_last && l == l_last) {
+                const bool odd = ((lt - 1 + l) & 1) != 0;
+                float v = odd ? DlA[0] : DlB[0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) v = (r == r_last) ? (odd ? DlA[r] : DlB[r]) : v;
+                gscore = v;
             }
-            if (s < steps) step(integral_constant<int, 0>{}, s, DlA, DlB, cA);
         }
         flush();
         // the row buffer written by this strip's last lane is read next strip

@@ -230,3 +230,69 @@ def test_dtw_central_differences():
         tol = 1e-3 + 16 * float(np.spacing(np.float32(max(abs(fp), abs(fm))))) / (2 * eps)
         assert abs((fp - fm) / (2 * eps) - gx[k, i, c]) < tol, (k, i, c, (fp - fm) / (2 * eps), gx[k, i, c])
     assert stable >= 12
+
+
+def test_tapes_share_the_solvers_buffers():
+    """Freed tapes hand their device buffers back to the solver.  Several live
+    tapes, freed in either order between forwards of different sizes, give
+    the gradients of fresh solvers; a tape whose solver went first frees
+    without touching it and refuses a backward."""
+    import ctypes
+    from pyalign_amd import algorithm, torch_ops
+    from pyalign_amd._lib import check, lib
+    rng = np.random.default_rng(64)
+    batches = [c4_batch(rng, n, lo, hi, dim=16) for n, lo, hi in ((30, 20, 120), (50, 10, 60), (20, 100, 200))]
+
+    def grads(S, ls, lt):
+        Sg = torch.tensor(S, device="cuda", requires_grad=True)
+        align_score(Sg, ls, lt, 1.0, 0.2, locality="global").sum().backward()
+        return Sg.grad.cpu().numpy()
+
+    torch_ops._SOLVERS.clear()
+    fresh = []
+    for b in batches:
+        fresh.append(grads(*b))
+        torch_ops._SOLVERS.clear()
+    # one solver (limits 256 x 256) for all: live tapes overlap
+    keep = []
+    for order in ((0, 1, 2), (2, 0, 1)):
+        live = []
+        for k in order:
+            S, ls, lt = batches[k]
+            Sg = torch.tensor(np.pad(S, ((0, 0), (0, 256 - S.shape[1]), (0, 256 - S.shape[2]))), device="cuda",
+                              requires_grad=True)
+            ls2, lt2 = ls.copy(), lt.copy()
+            ls2[0], lt2[0] = 256, 256  # the same rounded limits for every batch
+            Sg.data[0] = torch.randn(256, 256, device="cuda") / 8
+            out = align_score(Sg, ls2, lt2, 1.0, 0.2, locality="global")
+            live.append((k, Sg, out))
+        for q in (reversed(range(len(live))) if order[0] else range(len(live))):
+            k, Sg, out = live[q]
+            live[q] = None
+            out.sum().backward()
+            got = Sg.grad.cpu().numpy()[1:, :batches[k][0].shape[1], :batches[k][0].shape[2]]
+            assert np.array_equal(got.view(np.uint32), fresh[k][1:].view(np.uint32)), (order, k)
+            del out  # frees the tape: its buffers go back to the solver
+        keep.append(len(torch_ops._SOLVERS))
+    assert keep == [1, 1]
+    # a tape that outlives its solver
+    S, ls, lt = batches[1]
+    opts = algorithm.m16.create_options(dict(
+        solver=algorithm.Type.ALIGNMENT, batch=True, direction=algorithm.Direction.MAXIMIZE, return_dup=False,
+        codomain=pa.solve.Codomain(pa.Solution), locality=algorithm.Locality.GLOBAL,
+        gap_cost=pa.AffineGapCost(1.0, 0.2)))
+    solver = algorithm.m16.create_solver(64, 64, opts)
+    St = torch.tensor(S, device="cuda")
+    rg, hold = algorithm.make_ragged("dense", ls, lt, dense=St.data_ptr(),
+                                     dense_off=np.arange(len(ls), dtype=np.int64) * S.shape[1] * S.shape[2],
+                                     dense_ld=S.shape[2], on_device=True)
+    scores = torch.empty(len(ls), device="cuda")
+    h = ctypes.c_void_p()
+    check(lib().pa_forward_ragged(solver._h, ctypes.byref(rg), ctypes.c_void_p(scores.data_ptr()), ctypes.byref(h)))
+    check(lib().pa_solver_sync(solver._h))
+    del solver
+    g = torch.ones(len(ls), device="cuda")
+    dS = torch.zeros_like(St)
+    with pytest.raises(ValueError, match="solver was freed"):
+        check(lib().pa_backward(h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(dS.data_ptr()), None, None, None))
+    lib().pa_tape_free(h)
