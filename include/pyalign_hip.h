@@ -196,6 +196,15 @@ int pa_solver_last_schedule(const pa_solver *solver, int32_t *groups, int32_t *b
  * hipGraph and replayed; enable = 0 launches every kernel eagerly (also the
  * default when PYALIGN_AMD_NO_GRAPHS is set).  No reference counterpart. */
 int pa_solver_set_graphs(pa_solver *solver, int32_t enable);
+/* The solver's device work goes to `stream` (a hipStream_t; NULL is the
+ * device's null stream) when external is 1, and to the solver's own stream
+ * (the default) when it is 0.  On a caller's stream, pa_forward_ragged and
+ * pa_backward return without waiting for the device: their results are
+ * ordered on that stream like any other kernel's (a forward that ran problems
+ * in band mode still needs pa_solver_sync to report a stalled strip
+ * pipeline), and launch sequences are not captured as graphs.  A tape's
+ * forward and backward must use the same stream.  No reference counterpart. */
+int pa_solver_set_stream(pa_solver *solver, void *stream, int32_t external);
 /* Checkpointed differentiation: in pa_forward_ragged, a global problem on a
  * device-resident dense or series payload whose longer side exceeds min_len
  * (default 2048) keeps only the bottom row of each strip of 512 rows, not
@@ -223,7 +232,9 @@ int pa_solver_last_bad_symbol(const pa_solver *solver, int64_t *pos_s, int64_t *
  * are accumulated into (+=) and have the forward payload's layout; grad_gap
  * is float32[n][2] (open, extend; linear gap costs use the open slot for
  * d/d(gap per symbol)).  Pass NULL for gradients not wanted.  The tape
- * borrows the forward's payload and its solver: both must outlive it.
+ * borrows the forward's payload, which must outlive it, and the solver's
+ * device buffers, which pa_tape_free hands back; a tape whose solver was
+ * freed first can only be freed (pa_backward fails).
  */
 typedef struct pa_tape pa_tape;
 int pa_forward_ragged(pa_solver *solver, const pa_ragged *batch, float *scores_device, pa_tape **tape);

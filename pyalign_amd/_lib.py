@@ -65,6 +65,7 @@ SIGNATURES = {
                                                 ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),
     "pa_solver_set_graphs": (ctypes.c_int, [_P, _I32]),
     "pa_solver_set_checkpoint": (ctypes.c_int, [_P, _I64]),
+    "pa_solver_set_stream": (ctypes.c_int, [_P, _P, _I32]),
     "pa_solver_set_gap_costs": (ctypes.c_int, [_P, _P, _I64, _P, _I64]),
     "pa_solver_last_bad_symbol": (ctypes.c_int, [_P, _P, _P]),
     "pa_solver_last_schedule": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),

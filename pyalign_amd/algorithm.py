@@ -538,6 +538,12 @@ class Solver:
         strip in the backward instead of keeping their traceback (-1: never)."""
         check(lib().pa_solver_set_checkpoint(self._h, int(min_len)))
 
+    def set_stream(self, stream):
+        """Run on the caller's HIP stream (an int handle, 0 = null stream), or
+        on the solver's own stream when stream is None."""
+        ext = stream is not None
+        check(lib().pa_solver_set_stream(self._h, ctypes.c_void_p(int(stream) if ext else 0), int(ext)))
+
     def set_graphs(self, enable):
         """Replay repeated launch sequences from a captured hipGraph (default on)."""
         check(lib().pa_solver_set_graphs(self._h, int(bool(enable))))

@@ -149,6 +149,7 @@ struct Plan {
 // planning and the descriptor upload.
 struct PlanCache {
     bool valid = false;
+    bool uploaded = false;  // b_probs / b_groups / b_band_probs hold this plan
     std::vector<int64_t> key;
     std::vector<int32_t> len_s, len_t;
     std::vector<int64_t> off_s, off_t;
@@ -164,7 +165,9 @@ struct pa_solver {
     int64_t lim_s = 0, lim_t = 0;
     std::vector<float> border_s, border_t;
     float go_s = 0, ge_s = 0, go_t = 0, ge_t = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the solver's work goes: own_stream or the caller's
+    hipStream_t own_stream = nullptr;
+    bool external_stream = false;      // pa_solver_set_stream: results ordered on the caller's stream
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cus = 256;
     bool has_device = false;
@@ -180,6 +183,13 @@ struct pa_solver {
     DevBuf b_safe, b_sink;  // KArgs::safe, KArgs::sink
     int64_t last_bad[2] = {-1, -1};  // first symbol outside the alphabet, per side
     uint32_t *band_err_host = nullptr;  // pinned: the last band-mode launch's failure flag
+    // pinned staging of the plan upload: from pageable memory hipMemcpyAsync
+    // would wait for the stream; staged, the host returns at once, and the
+    // next upload waits only for the previous one's copies (stage_ev)
+    char *stage = nullptr;
+    size_t stage_cap = 0;
+    hipEvent_t stage_ev = nullptr;
+    bool stage_pending = false;
     bool band_check = false;            // a band-mode flag is on its way to band_err_host
     // launch sequences replayed from a hipGraph when they repeat
     bool use_graphs = true;
@@ -272,9 +282,12 @@ pa_solver::~pa_solver() {
     for (pa_tape *t : tapes) t->solver = nullptr;
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (band_err_host) (void)hipHostFree(band_err_host);
+    if (stage_pending) (void)hipEventSynchronize(stage_ev);
+    if (stage) (void)hipHostFree(stage);
+    if (stage_ev) (void)hipEventDestroy(stage_ev);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
 namespace {
@@ -337,9 +350,11 @@ void ensure_device(pa_solver &s) {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     s.cus = prop.multiProcessorCount;
-    HIPCHK(hipStreamCreate(&s.stream));
+    HIPCHK(hipStreamCreate(&s.own_stream));
+    s.stream = s.own_stream;
     HIPCHK(hipEventCreate(&s.ev0));
     HIPCHK(hipEventCreate(&s.ev1));
+    HIPCHK(hipEventCreateWithFlags(&s.stage_ev, hipEventDisableTiming));
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&s.band_err_host), 4, hipHostMallocDefault));
     *s.band_err_host = 0;
     s.b_border_s.ensure(s.border_s.size() * 4);
@@ -775,6 +790,7 @@ void run_wsb(pa_solver &s, const Input &in, const Job &job, std::unique_ptr<pa_r
         cells += double(pr.len_s) * double(pr.len_t);
     }
     upload(st, s.b_probs, probs.data(), probs.size());
+    s.last_plan.uploaded = false;  // b_probs now holds these descriptors
     upload(st, s.b_ws, s.ws_sgn.data(), s.ws_sgn.size());
     upload(st, s.b_wt, s.wt_sgn.data(), s.wt_sgn.size());
     const void *codes_s = nullptr, *codes_t = nullptr;
@@ -1009,9 +1025,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             }
             pl.cells += double(pr.len_s) * double(pr.len_t);
         }
-        upload(st, s.b_probs, pl.probs.data(), pl.probs.size());
-        upload(st, s.b_groups, pl.groups.data(), pl.groups.size());
-        if (!pl.band_probs.empty()) upload(st, s.b_band_probs, pl.band_probs.data(), pl.band_probs.size());
+        pc.uploaded = false;
         pc.key = key;
         pc.len_s.assign(in.len_s, in.len_s + n);
         pc.len_t.assign(in.len_t, in.len_t + n);
@@ -1020,6 +1034,34 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         pc.off_t.clear();
         if (o_t) pc.off_t.assign(o_t, o_t + n);
         pc.valid = true;
+    }
+    if (!pc.uploaded) {
+        const Plan &pl = pc.plan;
+        const size_t nb[3] = {pl.probs.size() * sizeof(Prob), pl.groups.size() * sizeof(Group),
+                              pl.band_probs.size() * sizeof(int32_t)};
+        const size_t al = 256, total = (nb[0] + al) + (nb[1] + al) + (nb[2] + al);
+        if (s.stage_pending) HIPCHK(hipEventSynchronize(s.stage_ev));
+        s.stage_pending = false;
+        if (s.stage_cap < total) {
+            if (s.stage) HIPCHK(hipHostFree(s.stage));
+            s.stage = nullptr;
+            s.stage_cap = 0;
+            HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&s.stage), total + total / 4, hipHostMallocDefault));
+            s.stage_cap = total + total / 4;
+        }
+        const void *src[3] = {pl.probs.data(), pl.groups.data(), pl.band_probs.data()};
+        DevBuf *dst[3] = {&s.b_probs, &s.b_groups, &s.b_band_probs};
+        size_t at = 0;
+        for (int q = 0; q < 3; q++) {
+            dst[q]->ensure(std::max<size_t>(nb[q], 4));
+            if (!nb[q]) continue;
+            std::memcpy(s.stage + at, src[q], nb[q]);
+            HIPCHK(hipMemcpyAsync(dst[q]->p, s.stage + at, nb[q], hipMemcpyHostToDevice, st));
+            at += (nb[q] + al - 1) / al * al;
+        }
+        HIPCHK(hipEventRecord(s.stage_ev, st));
+        s.stage_pending = true;
+        pc.uploaded = true;
     }
     const Plan &plan = pc.plan;
 
@@ -1163,9 +1205,12 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
 
     HIPCHK(hipEventRecord(s.ev0, st));
     hipError_t le = hipSuccess;
-    if (s.use_graphs && s.graph_exec && gkey == s.graph_key) {
+    // graphs are captured on the solver's own stream only (a caller's stream
+    // may be the null stream, or under a capture of its own)
+    const bool graphs = s.use_graphs && !s.external_stream;
+    if (graphs && s.graph_exec && gkey == s.graph_key) {
         le = hipGraphLaunch(s.graph_exec, st);
-    } else if (s.use_graphs && gkey == s.prev_key) {
+    } else if (graphs && gkey == s.prev_key) {
         if (s.graph_exec) {
             (void)hipGraphExecDestroy(s.graph_exec);
             s.graph_exec = nullptr;
@@ -1235,7 +1280,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             tape->kargs = a;  // a.rowbuf now points into the tape's row buffer
             tape->band_slots = plan.band_slots;
         }
-        s.last_plan.valid = false;  // its descriptors moved to the tape
+        s.last_plan.uploaded = false;  // its descriptors moved to the tape
         return;
     }
     if (!result) {
@@ -1761,6 +1806,15 @@ int pa_solver_last_bad_symbol(const pa_solver *s, int64_t *pos_s, int64_t *pos_t
     });
 }
 
+int pa_solver_set_stream(pa_solver *s, void *stream, int32_t external) {
+    return guard([&] {
+        if (!s) fail(PA_EINVAL, "null solver");
+        ensure_device(*s);
+        s->external_stream = external != 0;
+        s->stream = s->external_stream ? static_cast<hipStream_t>(stream) : s->own_stream;
+    });
+}
+
 int pa_solver_set_checkpoint(pa_solver *s, int64_t min_len) {
     return guard([&] {
         if (!s) fail(PA_EINVAL, "null solver");
@@ -1949,7 +2003,9 @@ int pa_backward(pa_tape *t, const float *grad_scores_device, float *grad_dense_d
         hipStream_t st = t->solver->stream;
         HIPCHK(launch_backward(t->gap, form, t->dim, a, st));
         if (!t->ckpt.empty()) backward_checkpointed(*t, a, st);
-        HIPCHK(hipStreamSynchronize(st));
+        // on the caller's stream the gradients are ordered there; the
+        // solver's own stream is waited for
+        if (!t->solver->external_stream) HIPCHK(hipStreamSynchronize(st));
     });
 }
 
@@ -1957,9 +2013,9 @@ void pa_tape_free(pa_tape *t) {
     if (!t) return;
     if (pa_solver *s = t->solver) {
         s->tapes.erase(std::remove(s->tapes.begin(), s->tapes.end(), t), s->tapes.end());
-        // the larger of each pair stays with the solver (the tape's work is
-        // done: pa_backward synchronises, and the solver's later work is
-        // ordered after it on the same stream)
+        // the larger of each pair stays with the solver: the solver's later
+        // work is ordered after the tape's on the same stream, and hipFree
+        // of the smaller one waits for the device
         auto give_back = [](DevBuf &from, DevBuf &to) {
             if (from.cap <= to.cap) return false;
             std::swap(from.p, to.p);
@@ -1967,7 +2023,7 @@ void pa_tape_free(pa_tape *t) {
             return true;
         };
         // the cached plan's descriptors live in b_probs
-        if (give_back(t->probs, s->b_probs)) s->last_plan.valid = false;
+        if (give_back(t->probs, s->b_probs)) s->last_plan.uploaded = false;
         give_back(t->seeds, s->b_seeds);
         give_back(t->tb, s->b_tb);
         give_back(t->rowbuf, s->b_rowbuf);

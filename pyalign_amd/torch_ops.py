@@ -85,6 +85,19 @@ def _new_solver(kind, direction, locality, gap, gap_open, gap_extend, lim_s, lim
     return bits.create_solver(lim_s, lim_t, bits.create_options(opts))
 
 
+def _forward(solver, rg, scores, keep):
+    """The forward pass on torch's current stream: kernels are ordered after
+    the inputs' producers and before the scores' consumers, with no host
+    wait (except to check a band-mode run's strip pipeline)."""
+    solver.set_stream(torch.cuda.current_stream(scores.device).cuda_stream)
+    h = ctypes.c_void_p()
+    check(lib().pa_forward_ragged(solver._h, ctypes.byref(rg), ctypes.c_void_p(scores.data_ptr()), ctypes.byref(h)))
+    tape = _Tape(h, solver, keep)
+    if solver.last_schedule()[1] > 0:
+        check(lib().pa_solver_sync(solver._h))
+    return tape
+
+
 class _AlignScore(torch.autograd.Function):
     @staticmethod
     def forward(ctx, S, gap_open, gap_extend, len_s, len_t, locality, direction, gap, checkpoint):
@@ -102,12 +115,7 @@ class _AlignScore(torch.autograd.Function):
         rg, keep = algorithm.make_ragged("dense", ls, lt, dense=S.data_ptr(), dense_off=dense_off,
                                          dense_ld=Lt, on_device=True)
         scores = torch.empty(n, dtype=torch.float32, device=S.device)
-        torch.cuda.current_stream(S.device).synchronize()
-        h = ctypes.c_void_p()
-        check(lib().pa_forward_ragged(solver._h, ctypes.byref(rg), ctypes.c_void_p(scores.data_ptr()),
-                                      ctypes.byref(h)))
-        check(lib().pa_solver_sync(solver._h))
-        ctx.tape = _Tape(h, solver, (rg, keep, S))
+        ctx.tape = _forward(solver, rg, scores, (rg, keep, S))
         ctx.shape = S.shape
         ctx.gap = gap
         return scores
@@ -118,7 +126,7 @@ class _AlignScore(torch.autograd.Function):
         n = ctx.shape[0]
         dS = torch.zeros(ctx.shape, dtype=torch.float32, device=grad.device)
         dgap = torch.zeros((n, 2), dtype=torch.float32, device=grad.device)
-        torch.cuda.current_stream(grad.device).synchronize()
+        ctx.tape.solver.set_stream(torch.cuda.current_stream(grad.device).cuda_stream)
         check(lib().pa_backward(ctx.tape.h, ctypes.c_void_p(grad.data_ptr()), ctypes.c_void_p(dS.data_ptr()),
                                 None, None, ctypes.c_void_p(dgap.data_ptr())))
         d_open = dgap[:, 0].sum()
@@ -165,12 +173,7 @@ class _DTW(torch.autograd.Function):
                                          off_s=np.arange(n, dtype=np.int64) * Lx,
                                          off_t=np.arange(n, dtype=np.int64) * Ly, dim=dim, on_device=True)
         scores = torch.empty(n, dtype=torch.float32, device=x.device)
-        torch.cuda.current_stream(x.device).synchronize()
-        h = ctypes.c_void_p()
-        check(lib().pa_forward_ragged(solver._h, ctypes.byref(rg), ctypes.c_void_p(scores.data_ptr()),
-                                      ctypes.byref(h)))
-        check(lib().pa_solver_sync(solver._h))
-        ctx.tape = _Tape(h, solver, (rg, keep, x, y))
+        ctx.tape = _forward(solver, rg, scores, (rg, keep, x, y))
         ctx.shapes = (x.shape, y.shape)
         return scores
 
@@ -179,7 +182,7 @@ class _DTW(torch.autograd.Function):
         grad = grad.contiguous().to(torch.float32)
         dx = torch.zeros(ctx.shapes[0], dtype=torch.float32, device=grad.device)
         dy = torch.zeros(ctx.shapes[1], dtype=torch.float32, device=grad.device)
-        torch.cuda.current_stream(grad.device).synchronize()
+        ctx.tape.solver.set_stream(torch.cuda.current_stream(grad.device).cuda_stream)
         check(lib().pa_backward(ctx.tape.h, ctypes.c_void_p(grad.data_ptr()), None,
                                 ctypes.c_void_p(dx.data_ptr()), ctypes.c_void_p(dy.data_ptr()), None))
         return dx, dy, None, None, None, None
