@@ -64,13 +64,39 @@ bool supported_R(int form, int tr, bool fastmax, int R) {
 // LDS per wavefront: a traceback tile of `tile_words` words (host-chosen:
 // small tiles and 4 wavefronts per block for many problems, one wavefront
 // with a large tile for a few long ones).
+#ifndef PA_WALK_KT_MAX
+#define PA_WALK_KT_MAX WAVE
+#endif
 struct WalkShape {
     int waves;       // wavefronts (problems) per block
     int tile_words;  // traceback words per wavefront tile
 };
 
-inline WalkShape walk_shape(int64_t n) {
-    return n >= 512 ? WalkShape{4, 2048} : WalkShape{1, 14336};
+inline int device_cus() {
+    static int cus = 0;
+    if (cus <= 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+// extra: LDS words a wavefront needs besides its tile (series windows).
+// Few problems: one wavefront per block with a large tile.  Otherwise four
+// per block, the tile as large as the wavefronts resident per CU allow
+// (larger tiles: fewer refills along a path).
+#ifndef PA_WALK_TILE_MAX
+#define PA_WALK_TILE_MAX 8192
+#endif
+inline WalkShape walk_shape(int64_t n, int extra) {
+    if (n < 512) return WalkShape{1, 14336};
+    const int64_t per_cu = (n + device_cus() - 1) / device_cus();
+    int tile = per_cu <= 4 ? 8192 : (per_cu <= 8 ? 4096 : 2048);
+    tile = tile < PA_WALK_TILE_MAX ? tile : PA_WALK_TILE_MAX;
+    while (tile > 2048 && size_t(4) * (tile + extra) * 4 > size_t(160) * 1024) tile /= 2;
+    return WalkShape{4, tile};
 }
 
 // A position (i, j) on one problem's traceback with its decomposition
@@ -79,12 +105,13 @@ inline WalkShape walk_shape(int64_t n) {
 struct TbWalker {
     uint32_t *lds;
     const uint32_t *tb;
-    int64_t tb_off, nsteps;
+    int64_t tb_off;
+    int nsteps;
     int L, R, wpc, bits, KT, ST;
     int t_st, k0, k1;  // loaded tile: strip, lanes [k0, k1), steps [s0, s1)
-    int64_t s0, s1;
+    int s0, s1;
     int i, j, st, k, r;
-    int64_t s;
+    int s;
 
     __device__ void init(uint32_t *l, int tile_words, const uint32_t *t, const Prob &pr, int b) {
         lds = l;
@@ -94,8 +121,11 @@ struct TbWalker {
         R = pr.R;
         bits = b;
         wpc = (b * R + 31) / 32;
-        nsteps = int64_t(pr.len_t) + L - 1;
-        KT = L < WAVE ? L : WAVE;
+        nsteps = pr.len_t + L - 1;
+        // tile shape: a path going diagonally leaves a lane every R rows, one
+        // step more than R, so tiles hold about R + 1 steps per lane
+        KT = L < PA_WALK_KT_MAX ? L : PA_WALK_KT_MAX;
+        while (KT > 1 && KT * KT * wpc * (R + 1) > tile_words) KT >>= 1;
         ST = tile_words / (KT * wpc);
         t_st = -1;
         k0 = k1 = 0;
@@ -109,7 +139,7 @@ struct TbWalker {
         const int rem = i - st * rps;
         k = rem / R;
         r = rem - k * R;
-        s = int64_t(j) + k;
+        s = j + k;
     }
     __device__ void up() {
         --i;
@@ -137,18 +167,31 @@ struct TbWalker {
             s0 = s1 > ST ? s1 - ST : 0;
             const int kw = (k1 - k0) * wpc;
             const int cnt = int(s1 - s0) * kw;
+            const uint32_t *base = tb + tb_off + (int64_t(t_st) * nsteps + s0) * L * wpc + int64_t(k0) * wpc;
+            const int sstride = L * wpc;
             __builtin_amdgcn_wave_barrier();  // the old tile's reads are done
-            for (int w = lane; w < cnt; w += WAVE) {
-                const int si = w / kw;
-                const int q = w - si * kw;
-                lds[w] = tb[tb_off + ((int64_t(t_st) * nsteps + s0 + si) * L + k0) * wpc + q];
+            // eight loads in flight per lane before their LDS stores
+            constexpr int U = 8;
+            for (int w0 = 0; w0 < cnt; w0 += U * WAVE) {
+                uint32_t v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int w = w0 + u * WAVE + lane;
+                    const int si = w / kw;
+                    v[u] = w < cnt ? base[int64_t(si) * sstride + (w - si * kw)] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int w = w0 + u * WAVE + lane;
+                    if (w < cnt) lds[w] = v[u];
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         const int bit = r * bits;
-        const uint32_t w = lds[(int(s - s0) * (k1 - k0) + (k - k0)) * wpc + (bit >> 5)];
+        const uint32_t w = lds[((s - s0) * (k1 - k0) + (k - k0)) * wpc + (bit >> 5)];
         return __builtin_amdgcn_readfirstlane((w >> (bit & 31)) & ((1u << bits) - 1u));
     }
 };
@@ -184,7 +227,10 @@ __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t
                                                       const int64_t *path_off, int32_t *path_len,
                                                       int tile_words) {
     extern __shared__ uint32_t smem[];
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    // the wave index read as a scalar: the problem, its descriptor and the
+    // whole walk state are then wave-uniform (scalar registers and scalar
+    // branches; the vector lanes only load tiles)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const int64_t k = int64_t(blockIdx.x) * (blockDim.x / WAVE) + wave;
     if (k >= n) return;  // whole wavefronts leave; no block-wide barriers follow
     const Prob pr = probs[k];
@@ -217,7 +263,7 @@ __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t
 hipError_t launch_walk(int gap, const Prob *probs, int64_t n, const uint32_t *tb, const int32_t *seeds,
                        int32_t *path, const int64_t *path_off, int32_t *path_len, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    const WalkShape ws = walk_shape(n);
+    const WalkShape ws = walk_shape(n, 0);
     const dim3 grid(unsigned((n + ws.waves - 1) / ws.waves)), block(ws.waves * WAVE);
     const size_t lds = size_t(ws.waves) * ws.tile_words * 4;
     if (gap == GAP_AFF)
@@ -249,6 +295,28 @@ constexpr int SERIES_WIN = 256;  // rows per series window (x and y each)
 // float add to global memory with no value returned (nothing waits on it)
 __device__ __forceinline__ void add_noret(float *p, float v) { (void)unsafeAtomicAdd(p, v); }
 
+// A walk's gradient adds, held one per lane and sent as one 64-lane
+// no-return atomic instruction per 64 adds.  A global memory operation in
+// every move would make the moves wait on it (the wait counter the walk's
+// tile loads use also counts the atomics); queued, a move touches only LDS.
+// Every lane of the wave calls push and flush (n is the same in all lanes).
+struct AddQueue {
+    float *p = nullptr;
+    float v = 0.0f;
+    int n = 0;
+    __device__ void push(float *ptr, float val, int lane) {
+        if (lane == n) {
+            p = ptr;
+            v = val;
+        }
+        if (++n == WAVE) flush(lane);
+    }
+    __device__ void flush(int lane) {
+        if (lane < n) add_noret(p, v);
+        n = 0;
+    }
+};
+
 template <int SD>
 struct RowWin {
     float *lds;        // SERIES_WIN * SD floats
@@ -279,7 +347,10 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_wor
     constexpr int SDD = SD > 0 ? SD : 1;
     constexpr int WIN = FORM == F_SERIES ? 2 * SERIES_WIN * SDD : 0;  // floats per wavefront
     extern __shared__ uint32_t smem[];
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    // the wave index read as a scalar: the problem, its descriptor and the
+    // whole walk state are then wave-uniform (scalar registers and scalar
+    // branches; the vector lanes only load tiles)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const int64_t k = int64_t(blockIdx.x) * (blockDim.x / WAVE) + wave;
     if (k >= a.n) return;
     const Prob pr = a.probs[k];
@@ -291,6 +362,7 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_wor
     uint32_t *mine = smem + size_t(wave) * (tile_words + WIN);
     TbWalker w;
     w.init(mine, tile_words, a.tb, pr, GAP == GAP_AFF ? 4 : 2);
+    AddQueue q;
     [[maybe_unused]] RowWin<SDD> wx, wy;
     // DTW series: the running sums of the current row / column
     [[maybe_unused]] float gx[SDD], gy[SDD];
@@ -302,20 +374,19 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_wor
     }
     auto flush_row = [&]() {
         if constexpr (FORM == F_SERIES) {
-            if (cur_i >= 0 && a.grad_xs && lane == 0)
-                for (int d = 0; d < SD; ++d) add_noret(a.grad_xs + (pr.off_s + cur_i) * SD + d, gx[d]);
+            if (cur_i >= 0 && a.grad_xs)
+                for (int d = 0; d < SD; ++d) q.push(a.grad_xs + (pr.off_s + cur_i) * SD + d, gx[d], lane);
         }
     };
     auto flush_col = [&]() {
         if constexpr (FORM == F_SERIES) {
-            if (cur_j >= 0 && a.grad_xt && lane == 0)
-                for (int d = 0; d < SD; ++d) add_noret(a.grad_xt + (pr.off_t + cur_j) * SD + d, gy[d]);
+            if (cur_j >= 0 && a.grad_xt)
+                for (int d = 0; d < SD; ++d) q.push(a.grad_xt + (pr.off_t + cur_j) * SD + d, gy[d], lane);
         }
     };
     auto cost_grad = [&](int i, int j) {
         if constexpr (FORM == F_DENSE) {
-            if (a.grad_dense && lane == 0)
-                add_noret(a.grad_dense + pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj, g);
+            if (a.grad_dense) q.push(a.grad_dense + pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj, g, lane);
         } else if constexpr (FORM == F_SERIES) {
             if (i != cur_i) {
                 flush_row();
@@ -370,17 +441,18 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_wor
             }
         }
     }
-    if (a.grad_gap && lane == 0) {
-        add_noret(a.grad_gap + 2 * out, a.gsign * g * opens);
-        add_noret(a.grad_gap + 2 * out + 1, a.gsign * g * extends);
+    if (a.grad_gap) {
+        q.push(a.grad_gap + 2 * out, a.gsign * g * opens, lane);
+        q.push(a.grad_gap + 2 * out + 1, a.gsign * g * extends, lane);
     }
+    q.flush(lane);
 }
 
 hipError_t launch_backward(int gap, int form, int dim, const BwArgs &args, hipStream_t st) {
     if (args.n == 0) return hipSuccess;
-    const WalkShape ws = walk_shape(args.n);
-    const dim3 grid(unsigned((args.n + ws.waves - 1) / ws.waves)), block(ws.waves * WAVE);
     const int win = form == F_SERIES ? 2 * SERIES_WIN * dim : 0;
+    const WalkShape ws = walk_shape(args.n, win);
+    const dim3 grid(unsigned((args.n + ws.waves - 1) / ws.waves)), block(ws.waves * WAVE);
     const size_t lds = size_t(ws.waves) * (ws.tile_words + win) * 4;
 #define PA_BW(G, F, D) \
     hipLaunchKernelGGL((pa_backward_kernel<G, F, D>), grid, block, lds, st, args, ws.tile_words)
@@ -500,7 +572,10 @@ __global__ __launch_bounds__(256) void pa_backward_strip_kernel(BwArgs a, StripW
     constexpr int SDD = SD > 0 ? SD : 1;
     constexpr int WIN = FORM == F_SERIES ? 2 * SERIES_WIN * SDD : 0;
     extern __shared__ uint32_t smem[];
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    // the wave index read as a scalar: the problem, its descriptor and the
+    // whole walk state are then wave-uniform (scalar registers and scalar
+    // branches; the vector lanes only load tiles)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     const int64_t k = int64_t(blockIdx.x) * (blockDim.x / WAVE) + wave;
     if (k >= a.n) return;
     const int st = strip_of[k];
@@ -514,6 +589,7 @@ __global__ __launch_bounds__(256) void pa_backward_strip_kernel(BwArgs a, StripW
     uint32_t *mine = smem + size_t(wave) * (tile_words + WIN);
     TbWalker w;
     w.init(mine, tile_words, a.tb, pr, GAP == GAP_AFF ? 4 : 2);
+    AddQueue q;
     [[maybe_unused]] RowWin<SDD> wx, wy;
     if constexpr (FORM == F_SERIES) {
         float *win = reinterpret_cast<float *>(mine + tile_words);
@@ -530,20 +606,19 @@ __global__ __launch_bounds__(256) void pa_backward_strip_kernel(BwArgs a, StripW
     int cur_i = ws.cur_i, cur_j = ws.cur_j;
     auto flush_row = [&]() {
         if constexpr (FORM == F_SERIES) {
-            if (cur_i >= 0 && a.grad_xs && lane == 0)
-                for (int d = 0; d < SD; ++d) add_noret(a.grad_xs + (pr.off_s + cur_i) * SD + d, gx[d]);
+            if (cur_i >= 0 && a.grad_xs)
+                for (int d = 0; d < SD; ++d) q.push(a.grad_xs + (pr.off_s + cur_i) * SD + d, gx[d], lane);
         }
     };
     auto flush_col = [&]() {
         if constexpr (FORM == F_SERIES) {
-            if (cur_j >= 0 && a.grad_xt && lane == 0)
-                for (int d = 0; d < SD; ++d) add_noret(a.grad_xt + (pr.off_t + cur_j) * SD + d, gy[d]);
+            if (cur_j >= 0 && a.grad_xt)
+                for (int d = 0; d < SD; ++d) q.push(a.grad_xt + (pr.off_t + cur_j) * SD + d, gy[d], lane);
         }
     };
     auto cost_grad = [&](int i, int j) {
         if constexpr (FORM == F_DENSE) {
-            if (a.grad_dense && lane == 0)
-                add_noret(a.grad_dense + pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj, g);
+            if (a.grad_dense) q.push(a.grad_dense + pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj, g, lane);
         } else if constexpr (FORM == F_SERIES) {
             if (i != cur_i) {
                 flush_row();
@@ -631,11 +706,12 @@ __global__ __launch_bounds__(256) void pa_backward_strip_kernel(BwArgs a, StripW
                 }
             }
         }
-        if (a.grad_gap && lane == 0) {
-            add_noret(a.grad_gap + 2 * out, a.gsign * g * opens);
-            add_noret(a.grad_gap + 2 * out + 1, a.gsign * g * extends);
+        if (a.grad_gap) {
+            q.push(a.grad_gap + 2 * out, a.gsign * g * opens, lane);
+            q.push(a.grad_gap + 2 * out + 1, a.gsign * g * extends, lane);
         }
     }
+    q.flush(lane);
     if (lane == 0) {
         ws.i = w.i;
         ws.j = w.j;
@@ -658,9 +734,9 @@ __global__ __launch_bounds__(256) void pa_backward_strip_kernel(BwArgs a, StripW
 hipError_t launch_backward_strips(int gap, int form, int dim, const BwArgs &args, StripWalk *state,
                                   const int32_t *strip_of, hipStream_t st) {
     if (args.n == 0) return hipSuccess;
-    const WalkShape ws = walk_shape(args.n);
-    const dim3 grid(unsigned((args.n + ws.waves - 1) / ws.waves)), block(ws.waves * WAVE);
     const int win = form == F_SERIES ? 2 * SERIES_WIN * dim : 0;
+    const WalkShape ws = walk_shape(args.n, win);
+    const dim3 grid(unsigned((args.n + ws.waves - 1) / ws.waves)), block(ws.waves * WAVE);
     const size_t lds = size_t(ws.waves) * (ws.tile_words + win) * 4;
 #define PA_BS(G, F, D)                                                                                         \
     hipLaunchKernelGGL((pa_backward_strip_kernel<G, F, D>), grid, block, lds, st, args, state, strip_of, \
