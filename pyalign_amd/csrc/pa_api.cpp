@@ -488,7 +488,7 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
     };
     const bool band_ok = job.tr != TR_ALL && supported_R(job.form, job.tr, false, BAND_R);
     std::vector<Item> items(static_cast<size_t>(n));
-    std::vector<Item> bands;
+    std::vector<Item> bands, ckpts;
     for (int64_t k = 0; k < n; k++) {
         int L = 64, R = Rs.back();
         double serial = 0.0;
@@ -498,9 +498,11 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         const int64_t rps = int64_t(L) * R;
         const int nst = ls[k] > 0 && lt[k] > 0 ? int((ls[k] + rps - 1) / rps) : 0;
         if (ckpt && ckpt[k]) {
-            int nb = 0;
-            const double sb = band_serial(ls[k], lt[k], &nb);
-            bands.push_back(Item{k, BAND_L, BAND_R, nb, lt[k], sb, true});
+            // checkpointed: one wavefront sweeps the strips of BAND_L x BAND_R
+            // rows in order (the layout the backward's strip recomputation
+            // uses), keeping every strip's bottom row and no traceback
+            const int64_t crs = int64_t(BAND_L) * BAND_R;
+            ckpts.push_back(Item{k, BAND_L, BAND_R, int((ls[k] + crs - 1) / crs), lt[k], 0.0, true});
             items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0, false};
             continue;
         }
@@ -515,7 +517,7 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         }
         items[size_t(k)] = Item{k, L, R, nst, lt[k], 0.0, false};
     }
-    if (!bands.empty())
+    if (!bands.empty() || !ckpts.empty())
         items.erase(std::remove_if(items.begin(), items.end(), [](const Item &it) { return it.L == 0; }),
                     items.end());
     // group identical layouts with similar shape: sort by R, L, strips, len_t
@@ -558,6 +560,29 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         g.R = R;
         g.cost = int64_t(double(g.max_nst) * double(g.max_lt + L - 1) * (STEP_OVERHEAD + R));
         gs.push_back(GroupR{g, job.fast ? 0 : R});
+    }
+    // checkpointed problems: a group each; band slots only for the
+    // backward's strip recomputation (progress counters and partials)
+    for (const Item &it : ckpts) {
+        Prob pr{};
+        pr.out = int32_t(it.k);
+        pr.L = BAND_L;
+        pr.R = BAND_R;
+        pr.band = int32_t(plan.band_slots);  // also: keep every strip's bottom row
+        pr.tb_off = NO_TB_OFF;
+        plan.band_slots += it.nst;
+        Group g{};
+        g.first = int32_t(plan.probs.size());
+        g.count = 1;
+        g.L = BAND_L;
+        g.lgL = ilog2(BAND_L);
+        g.max_lt = int32_t(it.lt);
+        g.max_nst = it.nst;
+        g.R = BAND_R;
+        g.strip = -1;
+        g.cost = int64_t(double(it.nst) * double(it.lt + BAND_L - 1) * (STEP_OVERHEAD + BAND_R));
+        gs.push_back(GroupR{g, job.fast ? 1000 + BAND_R : BAND_R});
+        plan.probs.push_back(pr);
     }
     // band-mode problems: one group per strip, strips in order, run by the
     // general kernel (its launch goes first: it holds the longest problems)
@@ -1118,6 +1143,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         a.progress = s.b_progress.as<uint32_t>();
         a.partials = s.b_partials.as<BandPartial>();
         a.band_err = s.b_band_err.as<uint32_t>();
+        a.band_sync = 1;
     }
 
     // ---- launches: specs first (occupancy queries stay outside any capture)
@@ -1159,12 +1185,12 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             }
         }
         const std::vector<int> okey = {spec.loc, spec.dir, spec.gap, spec.form, spec.tr, spec.R,
-                                       int(spec.fastmax), int(spec.lds_bytes), spec.pair};
+                                       int(spec.fastmax), int(spec.lds_bytes), spec.pair, in.dim};
         int waves_per_cu = 0;
         for (auto &e : s.occ_cache)
             if (e.first == okey) waves_per_cu = e.second;
         if (!waves_per_cu) {
-            waves_per_cu = std::max(1, occupancy_waves(spec));
+            waves_per_cu = std::max(1, occupancy_waves(spec, in.dim));
             s.occ_cache.push_back({okey, waves_per_cu});
         }
         // the fast kernel's large workgroups only where the groups fill every
@@ -1871,7 +1897,13 @@ namespace {
 void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
     pa_solver &s = *t.solver;
     const int bits = tb_bits(t.gap, TR_ONE);
-    const int64_t budget = int64_t(256) << 20;  // scratch traceback words per pass (1 GiB)
+    // scratch traceback words per pass: up to 6 GiB (a pass over every
+    // checkpointed pair at once on a large batch), at most half the free
+    // device memory, at least 256 MiB
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const int64_t budget = std::max<int64_t>(int64_t(64) << 20, std::min<int64_t>(int64_t(1536) << 20,
+                                                                                   int64_t(free_b / 8)));
     LaunchSpec spec;
     std::memset(&spec, 0, sizeof(spec));
     spec.loc = t.loc;
@@ -1880,7 +1912,7 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
     spec.form = t.form;
     spec.tr = TR_ONE;
     spec.R = BAND_R;
-    const int waves_per_cu = std::max(1, occupancy_waves(spec));
+    const int waves_per_cu = std::max(1, occupancy_waves(spec, t.dim));
     t.s_progress.ensure(size_t(std::max<int64_t>(t.band_slots, 1)) * 4);
     t.s_partials.ensure(size_t(std::max<int64_t>(t.band_slots, 1)) * sizeof(BandPartial));
     t.s_err.ensure(4);
@@ -1956,6 +1988,7 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
             k.progress = t.s_progress.as<uint32_t>();
             k.partials = t.s_partials.as<BandPartial>();
             k.band_err = t.s_err.as<uint32_t>();
+            k.band_sync = 0;  // the rows above were all kept by the forward
             const int64_t wg = (int64_t(groups.size()) + 3) / 4;
             spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg, int64_t(s.cus) * std::max(1, waves_per_cu / 4))));
             HIPCHK(launch_solve(spec, k, st));

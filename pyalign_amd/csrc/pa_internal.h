@@ -98,6 +98,8 @@ struct KArgs {
     uint32_t *progress;     // band mode: published columns per strip
     BandPartial *partials;  // band mode: seed search per strip
     uint32_t *band_err;     // band mode: set when a wait gave up
+    int32_t band_sync;      // band mode: 1 = strips of a problem run concurrently (progress
+                            // counters hand rows down); 0 = every row above is complete
     // SAFE_FLOATS readable floats: [0] = -inf, [1] = +inf, the rest 0; the
     // address a lane or row with nothing to read loads from
     const float *safe;
@@ -189,7 +191,8 @@ hipError_t launch_wsb(int loc, int dir, int form, const KArgs &args, int64_t n, 
                       int32_t *path, const int64_t *path_off, int32_t *path_len, int threads,
                       hipStream_t stream);
 bool supported_R(int form, int tr, bool fastmax, int R);
-int occupancy_waves(const LaunchSpec &spec);  // resident waves per CU for this instantiation
+// resident waves per CU for this instantiation (dim: series dimensions, selects the kernel)
+int occupancy_waves(const LaunchSpec &spec, int dim);
 
 // Traceback bits per cell and 32-bit words per lane per step.
 inline int tb_bits(int gap, int tr) { return tr == TR_ALL ? 8 : (tr == TR_ONE ? (gap == GAP_AFF ? 4 : 2) : 0); }

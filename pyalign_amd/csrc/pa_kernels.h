@@ -139,8 +139,11 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         const bool act = st < nst;
         const int row0 = st * rps + l * R;
         // row buffers: one reused row, or one row per strip boundary in band mode
-        const int64_t rb_in = BAND ? pr.rb_off + int64_t(st - 1) * lt : pr.rb_off;
-        const int64_t rb_out = BAND ? pr.rb_off + int64_t(st) * lt : pr.rb_off;
+        // a band slot (band mode, or a checkpointed forward) keeps every
+        // strip's bottom row; otherwise one row is handed down and reused
+        const bool keep_rows = BAND || pr.band >= 0;
+        const int64_t rb_in = keep_rows ? pr.rb_off + int64_t(st - 1) * lt : pr.rb_off;
+        const int64_t rb_out = keep_rows ? pr.rb_off + int64_t(st) * lt : pr.rb_off;
         [[maybe_unused]] uint32_t seen = 0;  // band mode: columns of strip st-1 known published
 
         int rowsym[R];
@@ -224,7 +227,8 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 for (int d = 0; d < SDD; ++d) c.v[d] = tbf[jc * SD + d];
             }
             if constexpr (BAND) {
-                if (l == 0 && act && st > 0 && j < lt) band_wait(a, pr.band + st - 1, uint32_t(j + 1), seen);
+                if (a.band_sync && l == 0 && act && st > 0 && j < lt)
+                    band_wait(a, pr.band + st - 1, uint32_t(j + 1), seen);
             }
             c.D = upD[jc * upDs];
             c.P = upP[jc * upPs];
@@ -483,14 +487,14 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                     pend_rb = a.rowbuf + rb_out + j;
                     pend_v = make_float2(botD, botP);
                     if constexpr (BAND) {
-                        pend_pub = ((j + 1) % BAND_PUB) == 0 || j == lt - 1;
+                        pend_pub = a.band_sync && (((j + 1) % BAND_PUB) == 0 || j == lt - 1);
                         pend_cols = uint32_t(j + 1);
                     }
                 }
 
                 if constexpr (TR != TR_SCORE) {
-                    // band-mode problems of a checkpointed forward keep no traceback
-                    if (!BAND || pr.tb_off != NO_TB_OFF) {
+                    // problems of a checkpointed forward keep no traceback
+                    if (pr.tb_off != NO_TB_OFF) {
                         pend_dst = a.tb + pr.tb_off + (int64_t(st) * nsteps_p + s) * int64_t(L) * WPC + l * WPC;
 #pragma unroll
                         for (int w = 0; w < WPC; ++w) pend_w[w] = tbw[w];

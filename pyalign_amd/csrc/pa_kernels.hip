@@ -27,9 +27,10 @@ hipError_t launch_solve(const LaunchSpec &s, const KArgs &a, hipStream_t st) {
     return dispatch(s, a, st, nullptr);
 }
 
-int occupancy_waves(const LaunchSpec &s) {
+int occupancy_waves(const LaunchSpec &s, int dim) {
     int waves = 0;
     KArgs dummy{};
+    dummy.dim = dim;
     if (dispatch(s, dummy, nullptr, &waves) != hipSuccess || waves <= 0) return 4;
     return waves;
 }
