@@ -181,6 +181,12 @@ struct pa_solver {
     // raw symbols encoded on the device (pa_ragged.raw_width)
     DevBuf b_raw_s, b_raw_t, b_lut, b_bad;
     DevBuf b_safe, b_sink;  // KArgs::safe, KArgs::sink
+    DevBuf b_ckpt_tb;       // checkpointed backward: recomputed tracebacks (scratch)
+    // checkpointed backward: scratch words per chunk when set (> 0), else
+    // sized from the free device memory
+    int64_t ckpt_words = std::getenv("PYALIGN_AMD_CKPT_SCRATCH_WORDS")
+                             ? std::atoll(std::getenv("PYALIGN_AMD_CKPT_SCRATCH_WORDS"))
+                             : 0;
     int64_t last_bad[2] = {-1, -1};  // first symbol outside the alphabet, per side
     uint32_t *band_err_host = nullptr;  // pinned: the last band-mode launch's failure flag
     // pinned staging of the plan upload: from pageable memory hipMemcpyAsync
@@ -275,7 +281,7 @@ struct pa_tape {
     std::vector<pa::Prob> ckpt;
     int64_t band_slots = 0;
     // backward scratch
-    DevBuf s_probs, s_groups, s_queue, s_tb, s_progress, s_partials, s_err, s_state, s_strip;
+    DevBuf s_probs, s_groups, s_queue, s_progress, s_partials, s_err, s_state, s_strip;
 };
 
 pa_solver::~pa_solver() {
@@ -1887,23 +1893,25 @@ int pa_forward_ragged(pa_solver *s, const pa_ragged *batch, float *scores_device
 namespace {
 
 // Backward over the checkpointed problems of a tape.  Their forward kept
-// only the bottom row of every strip (band mode, rows in the tape's row
-// buffer).  The walk runs one strip per pass, last strip first: each pass
-// recomputes that strip's one-optimal traceback into a scratch buffer with
-// the band kernel (row above from the kept rows; every wait satisfied by a
-// progress count of "all published") and then continues each problem's
-// walk through it (pa_backward_strip_kernel).  Problems are taken in chunks
-// whose one-strip tracebacks fit the scratch budget.
+// only the bottom row of every strip (in the tape's row buffer), so each
+// strip's traceback can be recomputed on its own from the row above (the
+// band kernel with no waits: every row above is complete).  Problems whose
+// whole recomputed traceback fits the scratch budget are done in chunks:
+// every strip of the chunk recomputed in one launch, then the ordinary
+// backward walk over them.  A problem too large for that walks one strip per
+// pass, last strip first: each pass recomputes that strip into the scratch
+// buffer and pa_backward_strip_kernel continues the walk through it.  The
+// scratch buffer belongs to the solver and is reused by its tapes.
 void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
     pa_solver &s = *t.solver;
     const int bits = tb_bits(t.gap, TR_ONE);
-    // scratch traceback words per pass: up to 6 GiB (a pass over every
-    // checkpointed pair at once on a large batch), at most half the free
-    // device memory, at least 256 MiB
+    // scratch traceback words: a quarter of the free device memory (plus
+    // what the solver's scratch already holds), at least 256 MiB
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    const int64_t budget = std::max<int64_t>(int64_t(64) << 20, std::min<int64_t>(int64_t(1536) << 20,
-                                                                                   int64_t(free_b / 8)));
+    const int64_t budget = s.ckpt_words > 0 ? s.ckpt_words
+                                            : std::max<int64_t>(int64_t(64) << 20,
+                                                                int64_t((free_b / 4 + s.b_ckpt_tb.cap) / 4));
     LaunchSpec spec;
     std::memset(&spec, 0, sizeof(spec));
     spec.loc = t.loc;
@@ -1923,18 +1931,82 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
         return (int64_t(pr.len_t) + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
     };
     auto strips_of = [&](const Prob &pr) { return (pr.len_s + pr.L * pr.R - 1) / (pr.L * pr.R); };
+    std::vector<Prob> whole, by_strip;
+    for (const Prob &pr : t.ckpt) (words_of(pr) * strips_of(pr) <= budget ? whole : by_strip).push_back(pr);
+    auto recompute = [&](const std::vector<Prob> &probs, const std::vector<Group> &groups) {
+        upload(st, t.s_probs, probs.data(), probs.size());
+        upload(st, t.s_groups, groups.data(), groups.size());
+        HIPCHK(launch_zero_u32(t.s_queue.as<uint32_t>(), 1, st));
+        KArgs k = t.kargs;
+        k.probs = t.s_probs.as<Prob>();
+        k.groups = t.s_groups.as<Group>();
+        k.n_groups = int32_t(groups.size());
+        k.queue = t.s_queue.as<uint32_t>();
+        k.tb = s.b_ckpt_tb.as<uint32_t>();
+        k.seeds = nullptr;
+        k.scores = nullptr;
+        k.dvals = nullptr;
+        k.val_layers = 1;
+        k.progress = t.s_progress.as<uint32_t>();
+        k.partials = t.s_partials.as<BandPartial>();
+        k.band_err = t.s_err.as<uint32_t>();
+        k.band_sync = 0;  // the rows above were all kept by the forward
+        const int64_t wg = (int64_t(groups.size()) + 3) / 4;
+        spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg, int64_t(s.cus) * std::max(1, waves_per_cu / 4))));
+        HIPCHK(launch_solve(spec, k, st));
+    };
+    auto one_group = [&](int q, const Prob &pr, int strip) {
+        Group g;
+        std::memset(&g, 0, sizeof(g));
+        g.first = q;
+        g.count = 1;
+        g.L = pr.L;
+        g.lgL = ilog2(pr.L);
+        g.max_lt = pr.len_t;
+        g.max_nst = strips_of(pr);
+        g.R = pr.R;
+        g.strip = strip;
+        return g;
+    };
+    // whole paths: every strip of a chunk at once, then the ordinary walk
+    size_t w0 = 0;
+    while (w0 < whole.size()) {
+        size_t w1 = w0;
+        int64_t acc = 0;
+        std::vector<Prob> probs;
+        std::vector<Group> groups;
+        while (w1 < whole.size() && (w1 == w0 || acc + words_of(whole[w1]) * strips_of(whole[w1]) <= budget)) {
+            Prob pr = whole[w1++];
+            pr.tb_off = acc;  // the forward's layout, in the scratch buffer
+            acc += words_of(pr) * strips_of(pr);
+            const int q = int(probs.size());
+            for (int sp = 0; sp < strips_of(pr); sp++) groups.push_back(one_group(q, pr, sp));
+            probs.push_back(pr);
+        }
+        s.b_ckpt_tb.ensure(size_t(std::max<int64_t>(acc, 1)) * 4);
+        // longest problems first (their strips are queued in strip order)
+        recompute(probs, groups);
+        BwArgs bw = base;
+        bw.probs = t.s_probs.as<Prob>();
+        bw.n = int64_t(probs.size());
+        bw.tb = s.b_ckpt_tb.as<uint32_t>();
+        HIPCHK(launch_backward(t.gap, t.form, t.dim, bw, st));
+        HIPCHK(hipStreamSynchronize(st));  // the descriptors are uploaded again by the next chunk
+        w0 = w1;
+    }
+    // too large for a whole path: one strip per pass
     size_t c0 = 0;
-    while (c0 < t.ckpt.size()) {
+    while (c0 < by_strip.size()) {
         size_t c1 = c0;
         int64_t words = 0;
-        while (c1 < t.ckpt.size() && (c1 == c0 || words + words_of(t.ckpt[c1]) <= budget)) words += words_of(t.ckpt[c1++]);
+        while (c1 < by_strip.size() && (c1 == c0 || words + words_of(by_strip[c1]) <= budget)) words += words_of(by_strip[c1++]);
         const int nc = int(c1 - c0);
         std::vector<int64_t> base_off(static_cast<size_t>(nc));
         std::vector<StripWalk> init(static_cast<size_t>(nc));
         int max_nst = 0;
         int64_t acc = 0;
         for (int q = 0; q < nc; q++) {
-            const Prob &pr = t.ckpt[c0 + size_t(q)];
+            const Prob &pr = by_strip[c0 + size_t(q)];
             base_off[size_t(q)] = acc;
             acc += words_of(pr);
             max_nst = std::max(max_nst, strips_of(pr));
@@ -1945,57 +2017,27 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
             w.cur_i = w.cur_j = -1;
             init[size_t(q)] = w;
         }
-        t.s_tb.ensure(size_t(std::max<int64_t>(acc, 1)) * 4);
+        s.b_ckpt_tb.ensure(size_t(std::max<int64_t>(acc, 1)) * 4);
         upload(st, t.s_state, init.data(), init.size());
         for (int it = 0; it < max_nst; it++) {
-            std::vector<Prob> probs(t.ckpt.begin() + int64_t(c0), t.ckpt.begin() + int64_t(c1));
+            std::vector<Prob> probs(by_strip.begin() + int64_t(c0), by_strip.begin() + int64_t(c1));
             std::vector<int32_t> strip_of(static_cast<size_t>(nc), -1);
             std::vector<Group> groups;
             for (int q = 0; q < nc; q++) {
                 Prob &pr = probs[size_t(q)];
-                const int nst = strips_of(pr);
-                const int sp = nst - 1 - it;
+                const int sp = strips_of(pr) - 1 - it;
                 if (sp < 0) continue;
                 // the strip's words land at this problem's scratch slot
                 pr.tb_off = base_off[size_t(q)] - int64_t(sp) * words_of(pr);
                 strip_of[size_t(q)] = sp;
-                Group g;
-                std::memset(&g, 0, sizeof(g));
-                g.first = q;
-                g.count = 1;
-                g.L = pr.L;
-                g.lgL = ilog2(pr.L);
-                g.max_lt = pr.len_t;
-                g.max_nst = nst;
-                g.R = pr.R;
-                g.strip = sp;
-                groups.push_back(g);
+                groups.push_back(one_group(q, pr, sp));
             }
-            upload(st, t.s_probs, probs.data(), probs.size());
-            upload(st, t.s_groups, groups.data(), groups.size());
             upload(st, t.s_strip, strip_of.data(), strip_of.size());
-            HIPCHK(launch_zero_u32(t.s_queue.as<uint32_t>(), 1, st));
-            KArgs k = t.kargs;
-            k.probs = t.s_probs.as<Prob>();
-            k.groups = t.s_groups.as<Group>();
-            k.n_groups = int32_t(groups.size());
-            k.queue = t.s_queue.as<uint32_t>();
-            k.tb = t.s_tb.as<uint32_t>();
-            k.seeds = nullptr;
-            k.scores = nullptr;
-            k.dvals = nullptr;
-            k.val_layers = 1;
-            k.progress = t.s_progress.as<uint32_t>();
-            k.partials = t.s_partials.as<BandPartial>();
-            k.band_err = t.s_err.as<uint32_t>();
-            k.band_sync = 0;  // the rows above were all kept by the forward
-            const int64_t wg = (int64_t(groups.size()) + 3) / 4;
-            spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg, int64_t(s.cus) * std::max(1, waves_per_cu / 4))));
-            HIPCHK(launch_solve(spec, k, st));
+            recompute(probs, groups);
             BwArgs bw = base;
             bw.probs = t.s_probs.as<Prob>();
             bw.n = nc;
-            bw.tb = t.s_tb.as<uint32_t>();
+            bw.tb = s.b_ckpt_tb.as<uint32_t>();
             HIPCHK(launch_backward_strips(t.gap, t.form, t.dim, bw, t.s_state.as<StripWalk>(),
                                           t.s_strip.as<int32_t>(), st));
             HIPCHK(hipStreamSynchronize(st));  // the host arrays above are reused

@@ -1,10 +1,12 @@
 """Checkpointed differentiation on the GPU.
 
 Long global pairs keep only strip boundary rows in the forward; the
-backward recomputes one strip's traceback at a time (last strip first) and
-continues the walk through it.  Scores and every gradient must be
-bit-identical to the run that keeps the full traceback (the sums are taken
-in the same order), and the schedule must show the pairs checkpointed.
+backward recomputes their tracebacks from those rows: whole paths at once
+where the scratch budget allows, else one strip at a time (last strip
+first), continuing the walk through each.  Scores and every gradient must be
+bit-identical to the run that keeps the full traceback, with the default
+budget and with a budget small enough to force the strip-by-strip passes,
+and the schedule must show the pairs checkpointed.
 """
 
 import numpy as np
@@ -41,12 +43,28 @@ def dtw_run(X, Y, lx, ly, checkpoint):
     return d.detach().cpu().numpy(), Xt.grad.cpu().numpy(), Yt.grad.cpu().numpy(), sched
 
 
+# scratch words per chunk: the default (free memory), or less than the
+# whole traceback of the longer pairs (those walk strip by strip)
+SCRATCH = {"auto": None, "strips": "300000"}
+
+
+def with_scratch(monkeypatch, scratch, fn):
+    torch_ops._SOLVERS.clear()  # the budget is read when a solver is made
+    with monkeypatch.context() as m:
+        if SCRATCH[scratch]:
+            m.setenv("PYALIGN_AMD_CKPT_SCRATCH_WORDS", SCRATCH[scratch])
+        out = fn()
+    torch_ops._SOLVERS.clear()
+    return out
+
+
+@pytest.mark.parametrize("scratch", list(SCRATCH))
 @pytest.mark.parametrize("dim", [1, 3])
-def test_dtw_checkpointed_equals_full_traceback(dim):
+def test_dtw_checkpointed_equals_full_traceback(dim, scratch, monkeypatch):
     rng = np.random.default_rng(101 + dim)
     X, Y, lx, ly = walks(rng, 12, 300, 1700, dim)
     full = dtw_run(X, Y, lx, ly, -1)
-    ck = dtw_run(X, Y, lx, ly, 1000)
+    ck = with_scratch(monkeypatch, scratch, lambda: dtw_run(X, Y, lx, ly, 1000))
     assert full[3][2] == 0 and ck[3][2] > 0, (full[3], ck[3])
     for a, b in zip(full[:3], ck[:3]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
@@ -66,15 +84,16 @@ def align_run(S, ls, lt, checkpoint, gap):
     return out, sched
 
 
+@pytest.mark.parametrize("scratch", list(SCRATCH))
 @pytest.mark.parametrize("gap", ["affine", "linear"])
-def test_align_checkpointed_equals_full_traceback(gap):
+def test_align_checkpointed_equals_full_traceback(gap, scratch, monkeypatch):
     rng = np.random.default_rng(111)
     n = 8
     ls = rng.integers(200, 1400, n).astype(np.int32)
     lt = rng.integers(200, 1400, n).astype(np.int32)
     S = (rng.standard_normal((n, int(ls.max()), int(lt.max()))) / 4).astype(np.float32)
     full, sf = align_run(S, ls, lt, -1, gap)
-    ck, sc = align_run(S, ls, lt, 600, gap)
+    ck, sc = with_scratch(monkeypatch, scratch, lambda: align_run(S, ls, lt, 600, gap))
     assert sf[2] == 0 and sc[2] > 0, (sf, sc)
     for a, b in zip(full, ck):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
