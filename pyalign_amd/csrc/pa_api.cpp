@@ -281,7 +281,7 @@ struct pa_tape {
     std::vector<pa::Prob> ckpt;
     int64_t band_slots = 0;
     // backward scratch
-    DevBuf s_probs, s_groups, s_queue, s_progress, s_partials, s_err, s_state, s_strip;
+    DevBuf s_probs, s_groups, s_queue, s_progress, s_partials, s_err, s_state, s_strip, s_scores;
 };
 
 pa_solver::~pa_solver() {
@@ -1931,6 +1931,7 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
         return (int64_t(pr.len_t) + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
     };
     auto strips_of = [&](const Prob &pr) { return (pr.len_s + pr.L * pr.R - 1) / (pr.L * pr.R); };
+    t.s_scores.ensure(size_t(std::max<int64_t>(t.n, 1)) * 4);
     std::vector<Prob> whole, by_strip;
     for (const Prob &pr : t.ckpt) (words_of(pr) * strips_of(pr) <= budget ? whole : by_strip).push_back(pr);
     auto recompute = [&](const std::vector<Prob> &probs, const std::vector<Group> &groups) {
@@ -1944,7 +1945,7 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
         k.queue = t.s_queue.as<uint32_t>();
         k.tb = s.b_ckpt_tb.as<uint32_t>();
         k.seeds = nullptr;
-        k.scores = nullptr;
+        k.scores = t.s_scores.as<float>();  // a throwaway copy of the scores (whole-path groups write them)
         k.dvals = nullptr;
         k.val_layers = 1;
         k.progress = t.s_progress.as<uint32_t>();
@@ -1979,12 +1980,16 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
             Prob pr = whole[w1++];
             pr.tb_off = acc;  // the forward's layout, in the scratch buffer
             acc += words_of(pr) * strips_of(pr);
-            const int q = int(probs.size());
-            for (int sp = 0; sp < strips_of(pr); sp++) groups.push_back(one_group(q, pr, sp));
+            // one wavefront sweeps all strips in order, as in a forward that
+            // keeps the traceback (the kept rows are rewritten unchanged)
+            groups.push_back(one_group(int(probs.size()), pr, -1));
             probs.push_back(pr);
         }
         s.b_ckpt_tb.ensure(size_t(std::max<int64_t>(acc, 1)) * 4);
-        // longest problems first (their strips are queued in strip order)
+        // longest problems first
+        std::stable_sort(groups.begin(), groups.end(), [&](const Group &a, const Group &b) {
+            return int64_t(a.max_nst) * (a.max_lt + 63) > int64_t(b.max_nst) * (b.max_lt + 63);
+        });
         recompute(probs, groups);
         BwArgs bw = base;
         bw.probs = t.s_probs.as<Prob>();

@@ -291,7 +291,7 @@ hipError_t launch_walk(int gap, const Prob *probs, int64_t n, const uint32_t *tb
 // series rows the DTW cost needs come through an LDS window that, like the
 // traceback tile, ends at the current row and is refilled when left.
 // ---------------------------------------------------------------------------
-constexpr int SERIES_WIN = 256;  // rows per series window (x and y each)
+constexpr int SERIES_WIN = 64;  // rows per series window (x and y each; small: more resident walks)
 
 // float add to global memory with no value returned (nothing waits on it)
 __device__ __forceinline__ void add_noret(float *p, float v) { (void)unsafeAtomicAdd(p, v); }
