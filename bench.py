@@ -327,6 +327,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the long-pair and fwd+bwd measurements")
+    ap.add_argument("--only", choices=["fwd_bwd", "dtw_fwd_bwd"], default=None,
+                    help="only time one differentiable workload (5 steps / 1 step) and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported "
@@ -346,6 +348,9 @@ def main():
         torch.add(src, 0.0, out=dst)  # reads and writes calibrate_copy bytes once
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
+        return
+    if args.only:
+        print(json.dumps(c4_bench(5) if args.only == "fwd_bwd" else c5_bench(1)))
         return
     table, ls, lt, codes_s, codes_t, off_s, off_t = c2_inputs()
     n = len(ls)
