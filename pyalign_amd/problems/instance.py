@@ -141,8 +141,13 @@ class ProblemBag:
         self._dtype = problems[0].dtype
         if any(p.dtype != self._dtype for p in problems):
             raise ValueError("problems in a bag need to have same dtype")
-        self._form = Form(min(p.form.value for p in problems))
-        shapes = np.array([p.shape for p in problems])
+        # a problem's form is fixed by its class: one lookup per class
+        reps = {}
+        for p in problems:
+            reps.setdefault(type(p), p)
+        self._form = Form(min(p.form.value for p in reps.values()))
+        shapes = np.array([p.shape for p in problems], dtype=np.int64).reshape(len(problems), 2)
+        self._shapes = shapes
         self._max_shape = (np.max(shapes[:, 0]), np.max(shapes[:, 1]))
 
     def __len__(self):
@@ -155,6 +160,11 @@ class ProblemBag:
     @property
     def max_shape(self):
         return self._max_shape
+
+    @property
+    def shapes(self):
+        """int64 [n, 2]: every problem's (|s|, |t|)."""
+        return self._shapes
 
     @property
     def direction(self):

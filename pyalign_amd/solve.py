@@ -539,8 +539,8 @@ class Solver:
         kind = _KIND[name[name.index("for_"):]]
         problems = bag.problems
         with self._timings.measure("prepare"):
-            ls = np.array([p.shape[0] for p in problems], dtype=np.int32)
-            lt = np.array([p.shape[1] for p in problems], dtype=np.int32)
+            ls = bag.shapes[:, 0].astype(np.int32)
+            lt = bag.shapes[:, 1].astype(np.int32)
             text = None
             if bag.form in (Form.INDEXED_MATRIX_FORM, Form.BINARY_MATRIX_FORM) and problems:
                 payload, text = _text_payload(problems, ls, lt, bag.form == Form.INDEXED_MATRIX_FORM)
