@@ -444,11 +444,11 @@ bool choose_layout(int64_t ls, int64_t lt, const std::vector<int> &Rs, double ca
 // BAND_PUB)) steps instead of strips * (len_t + 63).
 constexpr int BAND_L = 64, BAND_R = 8;
 
-double band_serial(int64_t ls, int64_t lt, int *nst_out) {
-    const int64_t rps = int64_t(BAND_L) * BAND_R;
+double band_serial(int64_t ls, int64_t lt, int R, int *nst_out) {
+    const int64_t rps = int64_t(BAND_L) * R;
     const int64_t nst = (ls + rps - 1) / rps;
     *nst_out = int(nst);
-    return double(lt + BAND_L - 1 + (nst - 1) * (BAND_L + BAND_PUB)) * (STEP_OVERHEAD + BAND_R);
+    return double(lt + BAND_L - 1 + (nst - 1) * (BAND_L + BAND_PUB)) * (STEP_OVERHEAD + R);
 }
 
 struct Job {
@@ -514,9 +514,10 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         }
         if (!fit && band_ok && serial > crit) {
             int nb = 0;
-            const double sb = band_serial(ls[k], lt[k], &nb) * band_penalty;
+            const int bR = band_R(job.form, job.tr);
+            const double sb = band_serial(ls[k], lt[k], bR, &nb) * band_penalty;
             if (nb >= 2 && sb < serial) {
-                bands.push_back(Item{k, BAND_L, BAND_R, nb, lt[k], sb, false});
+                bands.push_back(Item{k, BAND_L, bR, nb, lt[k], sb, false});
                 items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0, false};
                 continue;
             }
@@ -597,7 +598,7 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         Prob pr{};
         pr.out = int32_t(it.k);
         pr.L = BAND_L;
-        pr.R = BAND_R;
+        pr.R = it.R;
         pr.band = int32_t(plan.band_slots);
         if (it.ckpt) pr.tb_off = NO_TB_OFF;
         plan.band_slots += it.nst;
@@ -610,12 +611,12 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
             g.lgL = ilog2(BAND_L);
             g.max_lt = int32_t(it.lt);
             g.max_nst = it.nst;
-            g.R = BAND_R;
+            g.R = it.R;
             g.strip = st;
             g.cost = int64_t(it.band);
-            // with the other groups of strip height BAND_R, or (fast jobs) in a
+            // with the other groups of its strip height, or (fast jobs) in a
             // general-kernel launch of its own, marked by 1000 + R
-            gs.push_back(GroupR{g, job.fast ? 1000 + BAND_R : BAND_R});
+            gs.push_back(GroupR{g, job.fast ? 1000 + it.R : it.R});
         }
         plan.probs.push_back(pr);
     }

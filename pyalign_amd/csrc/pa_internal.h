@@ -71,7 +71,10 @@ struct BandPartial {
     int32_t j2;      // semiglobal: last-row column
     int32_t pad;
 };
-constexpr int BAND_PUB = 32;                 // columns per progress publication
+#ifndef PA_BAND_PUB
+#define PA_BAND_PUB 32
+#endif
+constexpr int BAND_PUB = PA_BAND_PUB;        // columns per progress publication
 constexpr uint32_t BAND_SPIN_LIMIT = 1u << 20;  // polls before a wait gives up (about a second)
 
 struct KArgs {
@@ -191,6 +194,9 @@ hipError_t launch_wsb(int loc, int dir, int form, const KArgs &args, int64_t n, 
                       int32_t *path, const int64_t *path_off, int32_t *path_len, int threads,
                       hipStream_t stream);
 bool supported_R(int form, int tr, bool fastmax, int R);
+// strip height of band-mode strips (long problems): 4 where the general
+// kernel is built for it (more, shorter strips on concurrent wavefronts), else 8
+int band_R(int form, int tr);
 // resident waves per CU for this instantiation (dim: series dimensions, selects the kernel)
 int occupancy_waves(const LaunchSpec &spec, int dim);
 

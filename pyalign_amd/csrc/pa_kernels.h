@@ -974,6 +974,12 @@ static hipError_t dispatch_lg(const LaunchSpec &s, const KArgs &a, hipStream_t s
                 default: return hipErrorInvalidValue;
             }
         }
+        if (s.R == 4) {  // band-mode strips of a long problem (more, shorter strips)
+            if (s.form == F_TABLE && s.tr == TR_ONE) PA_GO(F_TABLE, TR_ONE, 4, false, 0);
+            if (s.form == F_BINARY && s.tr == TR_SCORE) PA_GO(F_BINARY, TR_SCORE, 4, false, 0);
+            if (s.form == F_BINARY && s.tr == TR_ONE) PA_GO(F_BINARY, TR_ONE, 4, false, 0);
+            return hipErrorInvalidValue;
+        }
         if (s.R != 8) return hipErrorInvalidValue;
         switch (s.form) {
             case F_TABLE:

@@ -35,6 +35,12 @@ int occupancy_waves(const LaunchSpec &s, int dim) {
     return waves;
 }
 
+int band_R(int form, int tr) {
+    const bool r4 = (form == F_TABLE && (tr == TR_SCORE || tr == TR_ONE)) ||
+                    (form == F_BINARY && (tr == TR_SCORE || tr == TR_ONE));
+    return r4 ? 4 : 8;
+}
+
 bool supported_R(int form, int tr, bool fastmax, int R) {
     if (form == F_TABLE && tr == TR_SCORE) {
         if (fastmax) {
