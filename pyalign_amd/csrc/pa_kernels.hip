@@ -251,19 +251,27 @@ __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t
     w.init(smem + wave * tile_words, tile_words, tb, pr, GAP == GAP_AFF ? 4 : 2);
     w.seek(u, v);
     int32_t *P = path + 2 * path_off[out];
-    if (lane == 0) {
-        P[0] = u;
-        P[1] = v;
-    }
+    // path cells are held one per lane (cell cnt in lane cnt % 64) and
+    // stored 64 at a time with one coalesced store, so no move waits on a
+    // store of its own (the wait counter is shared with the tile loads)
+    int pi = u, pj = v;  // cell 0 sits in lane 0
     int cnt = 1;
+    auto spill = [&](int upto) {  // cells [upto & ~63, upto)
+        const int base = (upto - 1) & ~(WAVE - 1);
+        if (base + lane < upto) {
+            P[2 * (base + lane)] = pi;
+            P[2 * (base + lane) + 1] = pj;
+        }
+    };
     while (w.i >= 0 && w.j >= 0) {
         if (walk_step<GAP>(w, lane) == 0) break;
-        if (lane == 0) {
-            P[2 * cnt] = w.i;
-            P[2 * cnt + 1] = w.j;
+        if (lane == (cnt & (WAVE - 1))) {
+            pi = w.i;
+            pj = w.j;
         }
-        ++cnt;
+        if ((++cnt & (WAVE - 1)) == 0) spill(cnt);
     }
+    if (cnt & (WAVE - 1)) spill(cnt);
     if (lane == 0) path_len[out] = cnt;
 }
 
