@@ -107,18 +107,24 @@ inline WalkShape walk_shape(int64_t n, int extra) {
 }
 
 // A position (i, j) on one problem's traceback with its decomposition
-// (strip st, lane k, row r of the lane, step s = j + k), moved up and left
-// without divisions, and the LDS tile the codes are read from.
+// (strip st, lane k, bit position of row r in the lane's words, step
+// s = j + k), moved up and left without divisions or multiplies, and the
+// LDS tile the codes are read from (lanes [k0, k1), steps [s0, s1) of strip
+// t_st, step-major).  li is the current cell's first word in the tile,
+// kept up to date by the moves and recomputed when a tile is loaded.  ga is
+// the cell's offset in a dense payload with strides (gsi, gsj).
 struct TbWalker {
     uint32_t *lds;
     const uint32_t *tb;
     int64_t tb_off;
     int nsteps;
     int L, R, wpc, bits, KT, ST;
-    int t_st, k0, k1;  // loaded tile: strip, lanes [k0, k1), steps [s0, s1)
+    int t_st, k0, k1;
     int s0, s1;
-    int i, j, st, k, r;
-    int s;
+    int kww;  // words per tile step: (k1 - k0) * wpc
+    int i, j, st, k, bit;
+    int s, li;
+    int64_t ga, gsi, gsj;
 
     __device__ void init(uint32_t *l, int tile_words, const uint32_t *t, const Prob &pr, int b) {
         lds = l;
@@ -137,6 +143,10 @@ struct TbWalker {
         t_st = -1;
         k0 = k1 = 0;
         s0 = s1 = 0;
+        kww = 0;
+        li = 0;
+        gsi = pr.si;
+        gsj = pr.sj;
     }
     __device__ void seek(int i_, int j_) {
         i = i_;
@@ -145,15 +155,20 @@ struct TbWalker {
         st = i >= 0 ? i / rps : -1;
         const int rem = i - st * rps;
         k = rem / R;
-        r = rem - k * R;
+        bit = (rem - k * R) * bits;
         s = j + k;
+        t_st = -1;  // the next code() loads a tile around this cell
+        ga = int64_t(i) * gsi + int64_t(j) * gsj;
     }
     __device__ void up() {
         --i;
-        if (--r < 0) {
-            r = R - 1;
+        ga -= gsi;
+        bit -= bits;
+        if (bit < 0) {  // row R-1 of the lane above, one step earlier
+            bit = (R - 1) * bits;
             --s;
-            if (--k < 0) {  // into the previous strip
+            li -= kww + wpc;
+            if (--k < 0) {  // into the previous strip (the tile is reloaded)
                 k = L - 1;
                 s += L;
                 --st;
@@ -163,6 +178,8 @@ struct TbWalker {
     __device__ void left() {
         --j;
         --s;
+        li -= kww;
+        ga -= gsj;
     }
     // traceback code of the current cell (i, j >= 0; the same in every lane)
     __device__ uint32_t code(int lane) {
@@ -172,8 +189,10 @@ struct TbWalker {
             k0 = k1 > KT ? k1 - KT : 0;
             s1 = s + 1;
             s0 = s1 > ST ? s1 - ST : 0;
-            const int kw = (k1 - k0) * wpc;
-            const int cnt = int(s1 - s0) * kw;
+            kww = (k1 - k0) * wpc;
+            li = (s - s0) * kww + (k - k0) * wpc;
+            const int kw = kww;
+            const int cnt = (s1 - s0) * kw;
             const uint32_t *base = tb + tb_off + (int64_t(t_st) * nsteps + s0) * L * wpc + int64_t(k0) * wpc;
             const int sstride = L * wpc;
             __builtin_amdgcn_wave_barrier();  // the old tile's reads are done
@@ -197,8 +216,7 @@ struct TbWalker {
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        const int bit = r * bits;
-        const uint32_t w = lds[((s - s0) * (k1 - k0) + (k - k0)) * wpc + (bit >> 5)];
+        const uint32_t w = lds[li + (bit >> 5)];
         return __builtin_amdgcn_readfirstlane((w >> (bit & 31)) & ((1u << bits) - 1u));
     }
 };
@@ -399,9 +417,9 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_wor
                 for (int d = 0; d < SD; ++d) q.push(a.grad_xt + (pr.off_t + cur_j) * SD + d, gy[d], lane);
         }
     };
-    auto cost_grad = [&](int i, int j) {
+    auto cost_grad = [&](int i, int j, int64_t ga) {  // ga: the cell's dense offset
         if constexpr (FORM == F_DENSE) {
-            if (a.grad_dense) q.push(a.grad_dense + pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj, g, lane);
+            if (a.grad_dense) q.push(a.grad_dense + pr.off_s + ga, g, lane);
         } else if constexpr (FORM == F_SERIES) {
             if (i != cur_i) {
                 flush_row();
@@ -428,11 +446,12 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_wor
         w.seek(u0, v0);
         while (w.i >= 0 && w.j >= 0) {
             const int pu = w.i, pv = w.j;
+            const int64_t pga = w.ga;
             const int src = walk_step<GAP>(w, lane);
-            if constexpr (GAP == GAP_DTW) cost_grad(pu, pv);
+            if constexpr (GAP == GAP_DTW) cost_grad(pu, pv, pga);
             if (src == 0) break;  // local: the path starts at a zero cell
             if (src == 1) {
-                if constexpr (GAP != GAP_DTW) cost_grad(pu, pv);
+                if constexpr (GAP != GAP_DTW) cost_grad(pu, pv, pga);
             } else if constexpr (GAP == GAP_AFF) {
                 opens += 1.0f;
                 extends += float((pu - w.i) + (pv - w.j) - 1);  // run length - 1
@@ -631,9 +650,9 @@ __global__ __launch_bounds__(256) void pa_backward_strip_kernel(BwArgs a, StripW
                 for (int d = 0; d < SD; ++d) q.push(a.grad_xt + (pr.off_t + cur_j) * SD + d, gy[d], lane);
         }
     };
-    auto cost_grad = [&](int i, int j) {
+    auto cost_grad = [&](int i, int j, int64_t ga) {  // ga: the cell's dense offset
         if constexpr (FORM == F_DENSE) {
-            if (a.grad_dense) q.push(a.grad_dense + pr.off_s + int64_t(i) * pr.si + int64_t(j) * pr.sj, g, lane);
+            if (a.grad_dense) q.push(a.grad_dense + pr.off_s + ga, g, lane);
         } else if constexpr (FORM == F_SERIES) {
             if (i != cur_i) {
                 flush_row();
@@ -676,14 +695,15 @@ __global__ __launch_bounds__(256) void pa_backward_strip_kernel(BwArgs a, StripW
             continue;
         }
         const int ci = w.i, cj = w.j;
+        const int64_t cga = w.ga;
         const int src = int(w.code(lane) & 3u);
-        if constexpr (GAP == GAP_DTW) cost_grad(ci, cj);
+        if constexpr (GAP == GAP_DTW) cost_grad(ci, cj, cga);
         if (src == 0) {  // a local start
             finished = true;
             break;
         }
         if (src == 1) {
-            if constexpr (GAP != GAP_DTW) cost_grad(ci, cj);
+            if constexpr (GAP != GAP_DTW) cost_grad(ci, cj, cga);
             w.up();
             w.left();
         } else if (src == 2) {
