@@ -3,6 +3,7 @@ problem at least 8 columns wide, unit column stride): results must equal the
 oracle bit for bit, and equal the per-cell load path of the same kernel
 (PYALIGN_AMD_NO_DENSE_STAGING) in scores, paths and gradients."""
 
+import typing
 import zlib
 
 import numpy as np
@@ -98,3 +99,31 @@ def test_differentiable_workload_gradients_identical(monkeypatch):
 
     for x, y in zip(run(True), run(False)):
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+@pytest.mark.parametrize("loc", list(LOCS))
+def test_staged_rows_all_optimal_alignments(loc, monkeypatch):
+    """All-optimal tracebacks (8 tie bits per cell) from staged rows equal the
+    per-cell path's: half-integer similarities make ties common, and the first
+    alignments of every problem's enumeration must agree in order and maps."""
+    rng = np.random.default_rng(zlib.crc32(repr((loc, "drows-all")).encode()))
+    ls, lt, off, dense = bag(rng, 24, 8, 40)
+    runs = []
+    for staged in (True, False):
+        with monkeypatch.context() as m:
+            if not staged:
+                m.setenv("PYALIGN_AMD_NO_DENSE_STAGING", "1")
+            solver = make_solver(loc, ("affine", 3, 1), "maximize", typing.Iterator[pa.Alignment], 64)
+        its = solver.solve_ragged("alignment_iterator", "dense", ls, lt, dense=dense, dense_off=off)
+        out = []
+        for it in its:
+            seq = []
+            for _ in range(6):
+                a = it.next()
+                if a is None:
+                    break
+                seq.append((np.float32(a.score).tobytes(), np.asarray(a.s_to_t).tobytes()))
+            out.append(seq)
+        runs.append(out)
+    assert runs[0] == runs[1]
+    assert sum(len(x) > 1 for x in runs[0]) > 0  # some problems have tied optima
