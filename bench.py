@@ -268,7 +268,7 @@ def c5_bench(steps):
 def traffic():
     """HBM bytes per launch of the score kernel from the committed PMC runs
     (FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known size)."""
-    path = os.path.join(ROOT, "profiles", "pa_fast_kernel_c2_traffic.json")
+    path = os.path.join(ROOT, "profiles", "pa_fast_kernel_local_gotoh_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
