@@ -127,3 +127,32 @@ def test_staged_rows_all_optimal_alignments(loc, monkeypatch):
         runs.append(out)
     assert runs[0] == runs[1]
     assert sum(len(x) > 1 for x in runs[0]) > 0  # some problems have tied optima
+
+
+@pytest.mark.parametrize("loc", ["global", "semiglobal"])
+def test_staged_rows_in_band_mode(loc, monkeypatch):
+    """Two long dense problems (band mode: strips on concurrent wavefronts,
+    rows staged through LDS) among short ones: scores and paths equal the
+    oracle's and the per-cell path's."""
+    rng = np.random.default_rng(zlib.crc32(repr((loc, "drows-band")).encode()))
+    ls = np.concatenate([rng.integers(10, 60, 150), [2600, 2900], rng.integers(10, 60, 150)]).astype(np.int32)
+    lt = np.concatenate([rng.integers(10, 60, 150), [900, 1000], rng.integers(10, 60, 150)]).astype(np.int32)
+    sizes = ls.astype(np.int64) * lt
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    dense = (rng.integers(-4, 5, int(sizes.sum())) / 2).astype(np.float32)
+    res = []
+    for staged in (True, False):
+        with monkeypatch.context() as m:
+            if not staged:
+                m.setenv("PYALIGN_AMD_NO_DENSE_STAGING", "1")
+            solver = make_solver(loc, ("affine", 3, 1), "maximize", pa.Solution, 4096)
+        res.append(solver.solve_ragged("solution", "dense", ls, lt, dense=dense, dense_off=off))
+        assert solver.last_schedule()[1] >= 1, solver.last_schedule()
+    for k in [0, 149, 150, 151, 152, len(ls) - 1]:
+        a, b = int(ls[k]), int(lt[k])
+        S = dense[off[k]:off[k] + a * b].reshape(a, b)
+        ref = O.OracleResult(O.make_config(locality=LOCS[loc][1], gap=("affine", 3, 1), goal=O.GOAL_ONE,
+                                           lim_s=4096, lim_t=4096), S)
+        for r in res:
+            assert np.float32(r[k].score).tobytes() == ref.score.tobytes(), k
+            assert np.array_equal(np.asarray(r[k].path, dtype=np.int32), ref.path()), k
