@@ -478,7 +478,10 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         choose_layout(ls[k], lt[k], Rs, 1e300, L, R, &cost);
         total += cost / 64.0;
     }
-    const double cap = std::max(256.0, total / (double(simds) * 4.0));
+#ifndef PA_CAP_DIV
+#define PA_CAP_DIV 8.0
+#endif
+    const double cap = std::max(256.0, total / (double(simds) * PA_CAP_DIV));
     // band mode only for a problem that would otherwise be the critical path
     // (serial time above the balanced per-SIMD total); its strips run in the
     // general kernel, several times slower per cell than the fast one
