@@ -395,7 +395,10 @@ int ilog2(int x) {
 // serial time stays under `cap` (which keeps several wavefronts of work per
 // SIMD and the longest problem off the critical path); problems that fit no
 // layout take the one with the least serial time.
-constexpr double STEP_OVERHEAD = 4.0;
+#ifndef PA_STEP_OVERHEAD
+#define PA_STEP_OVERHEAD 4.0
+#endif
+constexpr double STEP_OVERHEAD = PA_STEP_OVERHEAD;
 
 bool choose_layout(int64_t ls, int64_t lt, const std::vector<int> &Rs, double cap, int &L_out, int &R_out,
                    double *cost_out = nullptr, double *serial_out = nullptr) {
