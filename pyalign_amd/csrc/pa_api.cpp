@@ -167,6 +167,10 @@ struct pa_solver {
     float go_s = 0, ge_s = 0, go_t = 0, ge_t = 0;
     hipStream_t stream = nullptr;      // where the solver's work goes: own_stream or the caller's
     hipStream_t own_stream = nullptr;
+    // checkpointed backward: walks run on a second stream while the next
+    // chunk is recomputed; events order the two (created on first use)
+    hipStream_t aux_stream = nullptr;
+    std::vector<hipEvent_t> ck_events;
     bool external_stream = false;      // pa_solver_set_stream: results ordered on the caller's stream
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cus = 256;
@@ -294,6 +298,8 @@ pa_solver::~pa_solver() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (own_stream) (void)hipStreamDestroy(own_stream);
+    if (aux_stream) (void)hipStreamDestroy(aux_stream);
+    for (hipEvent_t e : ck_events) (void)hipEventDestroy(e);
 }
 
 namespace {
@@ -1909,7 +1915,7 @@ namespace {
 // pass, last strip first: each pass recomputes that strip into the scratch
 // buffer and pa_backward_strip_kernel continues the walk through it.  The
 // scratch buffer belongs to the solver and is reused by its tapes.
-void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
+void backward_checkpointed(pa_tape &t, const BwArgs &base, int form, hipStream_t st) {
     pa_solver &s = *t.solver;
     const int bits = tb_bits(t.gap, TR_ONE);
     // scratch traceback words: a quarter of the free device memory (plus
@@ -1976,36 +1982,100 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, hipStream_t st) {
         g.strip = strip;
         return g;
     };
-    // whole paths: every strip of a chunk at once, then the ordinary walk
-    size_t w0 = 0;
-    while (w0 < whole.size()) {
-        size_t w1 = w0;
-        int64_t acc = 0;
-        std::vector<Prob> probs;
-        std::vector<Group> groups;
-        while (w1 < whole.size() && (w1 == w0 || acc + words_of(whole[w1]) * strips_of(whole[w1]) <= budget)) {
-            Prob pr = whole[w1++];
-            pr.tb_off = acc;  // the forward's layout, in the scratch buffer
-            acc += words_of(pr) * strips_of(pr);
-            // one wavefront sweeps all strips in order, as in a forward that
-            // keeps the traceback (the kept rows are rewritten unchanged)
-            groups.push_back(one_group(int(probs.size()), pr, -1));
-            probs.push_back(pr);
+    // whole paths.  All of them are recomputed in one launch on st where
+    // the budget allows (one launch keeps the longest problem's serial time
+    // off the others'), else in chunks of at most half the budget, into
+    // alternating halves of the scratch buffer; each chunk's walk runs on a
+    // second stream while the next chunk is recomputed (a half is reused
+    // only after the walk two chunks back).  The problems that kept their
+    // traceback are walked on the second stream first, during the first
+    // recomputation.
+    if (!s.aux_stream) HIPCHK(hipStreamCreateWithFlags(&s.aux_stream, hipStreamNonBlocking));
+    hipStream_t aux = s.aux_stream;
+    struct Chunk {
+        size_t p0, np, g0, ng;
+        int64_t words;
+    };
+    std::vector<Chunk> chunks;
+    std::vector<Prob> allp;
+    std::vector<Group> allg;
+    {
+        int64_t total = 0;
+        for (const Prob &pr : whole) total += words_of(pr) * strips_of(pr);
+        const int64_t limit = total <= budget ? budget : budget / 2;
+        size_t w0 = 0;
+        while (w0 < whole.size()) {
+            Chunk c{allp.size(), 0, allg.size(), 0, 0};
+            while (w0 < whole.size() &&
+                   (c.np == 0 || c.words + words_of(whole[w0]) * strips_of(whole[w0]) <= limit)) {
+                Prob pr = whole[w0++];
+                pr.tb_off = c.words;  // the forward's layout, at the start of the chunk's half
+                c.words += words_of(pr) * strips_of(pr);
+                // one wavefront sweeps all strips in order, as in a forward that
+                // keeps the traceback (the kept rows are rewritten unchanged)
+                allg.push_back(one_group(int(c.np), pr, -1));
+                allp.push_back(pr);
+                c.np++;
+            }
+            c.ng = allg.size() - c.g0;
+            // longest problems first
+            std::stable_sort(allg.begin() + int64_t(c.g0), allg.end(), [&](const Group &x, const Group &y) {
+                return int64_t(x.max_nst) * (x.max_lt + 63) > int64_t(y.max_nst) * (y.max_lt + 63);
+            });
+            chunks.push_back(c);
         }
-        s.b_ckpt_tb.ensure(size_t(std::max<int64_t>(acc, 1)) * 4);
-        // longest problems first
-        std::stable_sort(groups.begin(), groups.end(), [&](const Group &a, const Group &b) {
-            return int64_t(a.max_nst) * (a.max_lt + 63) > int64_t(b.max_nst) * (b.max_lt + 63);
-        });
-        recompute(probs, groups);
-        BwArgs bw = base;
-        bw.probs = t.s_probs.as<Prob>();
-        bw.n = int64_t(probs.size());
-        bw.tb = s.b_ckpt_tb.as<uint32_t>();
-        HIPCHK(launch_backward(t.gap, t.form, t.dim, bw, st));
-        HIPCHK(hipStreamSynchronize(st));  // the descriptors are uploaded again by the next chunk
-        w0 = w1;
     }
+    const size_t nch = chunks.size();
+    int64_t half = 0;
+    for (const Chunk &c : chunks) half = std::max(half, c.words);
+    s.b_ckpt_tb.ensure(size_t(std::max<int64_t>(nch > 1 ? 2 * half : half, 1)) * 4);
+    while (s.ck_events.size() < 2 * nch + 2) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        s.ck_events.push_back(e);
+    }
+    hipEvent_t *ev_rec = s.ck_events.data() + 2, *ev_walk = s.ck_events.data() + 2 + nch;
+    if (!allp.empty()) {
+        upload(st, t.s_probs, allp.data(), allp.size());
+        upload(st, t.s_groups, allg.data(), allg.size());
+    }
+    t.s_queue.ensure(4 * (nch + 1));
+    HIPCHK(launch_zero_u32(t.s_queue.as<uint32_t>(), int64_t(nch + 1), st));
+    HIPCHK(hipEventRecord(s.ck_events[0], st));
+    HIPCHK(hipStreamWaitEvent(aux, s.ck_events[0], 0));
+    HIPCHK(launch_backward(t.gap, form, t.dim, base, aux));  // skips the checkpointed problems
+    for (size_t c = 0; c < nch; c++) {
+        const Chunk &ch = chunks[c];
+        if (c >= 2) HIPCHK(hipStreamWaitEvent(st, ev_walk[c - 2], 0));
+        uint32_t *tbh = s.b_ckpt_tb.as<uint32_t>() + (c & 1) * half;
+        KArgs k = t.kargs;
+        k.probs = t.s_probs.as<Prob>() + ch.p0;
+        k.groups = t.s_groups.as<Group>() + ch.g0;
+        k.n_groups = int32_t(ch.ng);
+        k.queue = t.s_queue.as<uint32_t>() + c;
+        k.tb = tbh;
+        k.seeds = nullptr;
+        k.scores = t.s_scores.as<float>();  // a throwaway copy of the scores
+        k.dvals = nullptr;
+        k.val_layers = 1;
+        k.progress = t.s_progress.as<uint32_t>();
+        k.partials = t.s_partials.as<BandPartial>();
+        k.band_err = t.s_err.as<uint32_t>();
+        k.band_sync = 0;
+        const int64_t wg = (int64_t(ch.ng) + 3) / 4;
+        spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg, int64_t(s.cus) * std::max(1, waves_per_cu / 4))));
+        HIPCHK(launch_solve(spec, k, st));
+        HIPCHK(hipEventRecord(ev_rec[c], st));
+        HIPCHK(hipStreamWaitEvent(aux, ev_rec[c], 0));
+        BwArgs bw = base;
+        bw.probs = t.s_probs.as<Prob>() + ch.p0;
+        bw.n = int64_t(ch.np);
+        bw.tb = tbh;
+        HIPCHK(launch_backward(t.gap, form, t.dim, bw, aux));
+        HIPCHK(hipEventRecord(ev_walk[c], aux));
+    }
+    HIPCHK(hipEventRecord(s.ck_events[1], aux));
+    HIPCHK(hipStreamWaitEvent(st, s.ck_events[1], 0));  // everything below is ordered after the walks
     // too large for a whole path: one strip per pass
     size_t c0 = 0;
     while (c0 < by_strip.size()) {
@@ -2088,8 +2158,8 @@ int pa_backward(pa_tape *t, const float *grad_scores_device, float *grad_dense_d
         const int form = (t->form == F_TABLE || t->form == F_BINARY) ? F_DENSE : t->form;
         if (t->form == F_TABLE || t->form == F_BINARY) a.grad_dense = nullptr;  // no similarity input to differentiate
         hipStream_t st = t->solver->stream;
-        HIPCHK(launch_backward(t->gap, form, t->dim, a, st));
-        if (!t->ckpt.empty()) backward_checkpointed(*t, a, st);
+        if (t->ckpt.empty()) HIPCHK(launch_backward(t->gap, form, t->dim, a, st));
+        else backward_checkpointed(*t, a, form, st);  // also walks the other problems
         // on the caller's stream the gradients are ordered there; the
         // solver's own stream is waited for
         if (!t->solver->external_stream) HIPCHK(hipStreamSynchronize(st));

@@ -43,9 +43,11 @@ def dtw_run(X, Y, lx, ly, checkpoint):
     return d.detach().cpu().numpy(), Xt.grad.cpu().numpy(), Yt.grad.cpu().numpy(), sched
 
 
-# scratch words per chunk: the default (free memory), or less than the
-# whole traceback of the longer pairs (those walk strip by strip)
-SCRATCH = {"auto": None, "strips": "300000"}
+# scratch words: the default (free memory: one recomputation launch), room
+# for a few whole paths at a time (chunks recomputed while the previous
+# chunk's walk runs on a second stream), or less than the whole traceback of
+# the longer pairs (those walk strip by strip)
+SCRATCH = {"auto": None, "chunks": "1000000", "strips": "300000"}
 
 
 def with_scratch(monkeypatch, scratch, fn):
