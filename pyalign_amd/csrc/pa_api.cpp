@@ -494,7 +494,10 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
     // band mode only for a problem that would otherwise be the critical path
     // (serial time above the balanced per-SIMD total); its strips run in the
     // general kernel, several times slower per cell than the fast one
-    const double crit = 4.0 * cap;
+#ifndef PA_BAND_CRIT
+#define PA_BAND_CRIT 4.0
+#endif
+    const double crit = PA_BAND_CRIT * cap;
     const double band_penalty = job.fast ? 3.0 : 1.0;
 
     struct Item {
