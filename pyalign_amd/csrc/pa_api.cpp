@@ -465,9 +465,9 @@ struct Job {
     bool fast;
 };
 
-// ckpt[k] != 0: problem k runs in band mode whatever its cost and keeps no
-// traceback (a checkpointed forward: the strip boundary rows are kept and
-// the backward recomputes one strip at a time).
+// ckpt[k] != 0: problem k keeps no traceback (a checkpointed forward): one
+// wavefront sweeps its strips of BAND_L x BAND_R rows in order and keeps
+// every strip's bottom row, from which the backward recomputes it.
 Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, int simds,
                const uint8_t *ckpt = nullptr) {
     std::vector<int> Rs;
@@ -505,7 +505,7 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         int L, R, nst;
         int64_t lt;
         double band;  // > 0: band mode, its serial time
-        bool ckpt;    // band mode for a checkpointed forward
+        bool ckpt;    // a checkpointed problem (strip rows kept, no traceback)
     };
     const bool band_ok = job.tr != TR_ALL && supported_R(job.form, job.tr, false, BAND_R);
     std::vector<Item> items(static_cast<size_t>(n));
