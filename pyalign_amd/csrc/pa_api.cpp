@@ -478,7 +478,8 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
             if (supported_R(job.form, job.tr, job.fast, R)) Rs.push_back(R);
     }
     // total wavefront time at the cheapest layouts (no cap) sets the cap:
-    // four wavefronts' worth of work per SIMD
+    // an eighth of a SIMD's share of the work for the fast kernel, a
+    // sixteenth for the general kernel (both measured best)
     double total = 0.0;
     for (int64_t k = 0; k < n; k++) {
         if (ls[k] <= 0 || lt[k] <= 0) continue;
@@ -490,7 +491,10 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
 #ifndef PA_CAP_DIV
 #define PA_CAP_DIV 8.0
 #endif
-    const double cap = std::max(256.0, total / (double(simds) * PA_CAP_DIV));
+#ifndef PA_CAP_DIV_GENERAL
+#define PA_CAP_DIV_GENERAL 16.0
+#endif
+    const double cap = std::max(256.0, total / (double(simds) * (job.fast ? PA_CAP_DIV : PA_CAP_DIV_GENERAL)));
     // band mode only for a problem that would otherwise be the critical path
     // (serial time above the balanced per-SIMD total); its strips run in the
     // general kernel, several times slower per cell than the fast one
