@@ -257,6 +257,15 @@ int pa_result_path(const pa_result *r, int64_t k, int32_t *uv);
 /* Alignment of problem k (factory.h:130-193): s_to_t int32[len_s],
  * t_to_s int32[len_t], -1 where unaligned. */
 int pa_result_alignment(const pa_result *r, int64_t k, int32_t *s_to_t, int32_t *t_to_s);
+/* All problems of a one-optimal result at once (a whole bag's Solution
+ * objects are built from two calls instead of two per problem; no reference
+ * counterpart).  pa_result_paths: lens[k] = problem k's path length (-1: no
+ * path); with uv non-NULL, the paths' (u, v) pairs concatenated in problem
+ * order.  pa_result_alignments: every problem's s_to_t (len_s[k] entries)
+ * and t_to_s (len_t[k] entries), concatenated in problem order, -1 for no
+ * path. */
+int pa_result_paths(const pa_result *r, int64_t *lens, int32_t *uv);
+int pa_result_alignments(const pa_result *r, int32_t *s_to_t, int32_t *t_to_s);
 /* All-optimal iteration (TracebackIterators<true>, solver.h:1568-1743):
  * returns 1 and fills score/path for the next alignment of problem k, 0 at
  * the end.  The path is read with pa_result_iter_path afterwards. */

@@ -85,6 +85,8 @@ SIGNATURES = {
     "pa_solve_ragged_matrices": (ctypes.c_int, [_P, ctypes.POINTER(Ragged), ctypes.POINTER(_P)]),
     "pa_result_layers": (_I32, [_P]),
     "pa_result_values": (ctypes.c_int, [_P, _I64, _P]),
+    "pa_result_paths": (ctypes.c_int, [_P, _P, _P]),
+    "pa_result_alignments": (ctypes.c_int, [_P, _P, _P]),
     "pa_result_traceback": (ctypes.c_int, [_P, _I64, _P]),
     "pa_result_edges_len": (_I64, [_P, _I64, _I32]),
     "pa_result_edges": (ctypes.c_int, [_P, _I64, _I32, _P]),

@@ -270,6 +270,22 @@ class _Result:
         check(lib().pa_result_path(self.h, k, uv.ctypes.data))
         return uv.astype(self.index_dtype)
 
+    def all_paths(self):
+        """Every problem's path length (-1: none) and the paths concatenated."""
+        lens = np.empty(max(self.n, 1), dtype=np.int64)
+        check(lib().pa_result_paths(self.h, lens.ctypes.data, None))
+        total = int(np.maximum(lens[:self.n], 0).sum())
+        uv = np.empty((max(total, 1), 2), dtype=np.int32)
+        check(lib().pa_result_paths(self.h, lens.ctypes.data, uv.ctypes.data))
+        return lens[:self.n], uv[:total].astype(self.index_dtype)
+
+    def all_alignments(self):
+        """Every problem's s_to_t and t_to_s, concatenated in problem order."""
+        s2t = np.empty(max(int(self.len_s.sum()), 1), dtype=np.int32)
+        t2s = np.empty(max(int(self.len_t.sum()), 1), dtype=np.int32)
+        check(lib().pa_result_alignments(self.h, s2t.ctypes.data, t2s.ctypes.data))
+        return s2t.astype(self.index_dtype), t2s.astype(self.index_dtype)
+
     def alignment(self, k, score):
         ls, lt = int(self.len_s[k]), int(self.len_t[k])
         s_to_t = np.empty(ls, dtype=np.int32)
@@ -367,18 +383,28 @@ class Solver:
         def lazy(k):
             if one is None:
                 return None
-            form, payload = one(k)
-            return _Matrices(self, form, r.len_s[k], r.len_t[k], payload)
+            a, b = int(r.len_s[k]), int(r.len_t[k])  # the closure keeps no result handle alive
+
+            def make():
+                form, payload = one(k)
+                return _Matrices(self, form, a, b, payload)
+
+            return _Deferred(make)
 
         if kind == "solution":
             alg = _algorithm_for(self._opts)
+            lens, uv = r.all_paths()
+            s2t, t2s = r.all_alignments()
+            po = np.concatenate([[0], np.cumsum(np.maximum(lens, 0))])
+            so = np.concatenate([[0], np.cumsum(r.len_s)])
+            to = np.concatenate([[0], np.cumsum(r.len_t)])
             out = []
             for k in range(n):
-                path = r.path(k)
-                if path is None:
+                if lens[k] < 0:
                     out.append(Solution(None, None, None, alg, lazy=lazy(k)))
                 else:
-                    out.append(Solution(float(scores[k]), path, r.alignment(k, scores[k]), alg, lazy=lazy(k)))
+                    al = Alignment(scores[k], s2t[so[k]:so[k + 1]], t2s[to[k]:to[k + 1]])
+                    out.append(Solution(float(scores[k]), uv[po[k]:po[k + 1]], al, alg, lazy=lazy(k)))
             return tuple(out)
         if kind == "alignment_iterator":
             return tuple(AlignmentIterator(r, k) for k in range(n))
@@ -571,10 +597,27 @@ class Solver:
         return ms.value, launches.value, cells.value
 
 
+class _Deferred:
+    """A problem's Solution matrices, built on first use."""
+
+    __slots__ = ("_make", "_m")
+
+    def __init__(self, make):
+        self._make, self._m = make, None
+
+    def __call__(self):
+        if self._m is None:
+            self._m = self._make()
+        return self._m()
+
+
 def _ragged_one(form, len_s, len_t, p):
-    """Problem k's data out of a ragged payload, as a one-problem payload."""
-    ls = np.asarray(len_s, dtype=np.int64)
-    lt = np.asarray(len_t, dtype=np.int64)
+    """Problem k's data out of a ragged payload, as a one-problem payload
+    (the payload's arrays are copied once here, so later changes by the
+    caller do not reach solutions built from them)."""
+    ls = np.array(len_s, dtype=np.int64)
+    lt = np.array(len_t, dtype=np.int64)
+    p = {key: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for key, v in p.items()}
 
     def one(k):
         a, b = int(ls[k]), int(lt[k])

@@ -2226,6 +2226,20 @@ int pa_result_path(const pa_result *r, int64_t k, int32_t *uv) {
     });
 }
 
+int pa_result_paths(const pa_result *r, int64_t *lens, int32_t *uv) {
+    return guard([&] {
+        if (!r || !lens || r->goal != PA_GOAL_ONE) fail(PA_EINVAL, "not a one-optimal result");
+        int64_t at = 0;
+        for (int64_t k = 0; k < r->n; k++) {
+            const int32_t m = r->path_len[size_t(k)];
+            lens[k] = m;
+            if (m <= 0) continue;
+            if (uv) std::memcpy(uv + 2 * at, r->paths.data() + 2 * r->path_off[size_t(k)], size_t(m) * 8);
+            at += m;
+        }
+    });
+}
+
 // path_compressor (solver.h:809-846) + Alignment::add_edge (factory.h:164-170)
 static void alignment_from_path(const int32_t *uv, int64_t m, int32_t ls, int32_t lt, int32_t *s_to_t,
                                 int32_t *t_to_s) {
@@ -2251,6 +2265,26 @@ static void alignment_from_path(const int32_t *uv, int64_t m, int32_t ls, int32_
         for (int32_t j = 0; j < lt; j++)
             if (t_to_s[j] >= 0) s_to_t[t_to_s[j]] = j;
     }
+}
+
+int pa_result_alignments(const pa_result *r, int32_t *s_to_t, int32_t *t_to_s) {
+    return guard([&] {
+        if (!r || !s_to_t || !t_to_s || r->goal != PA_GOAL_ONE) fail(PA_EINVAL, "not a one-optimal result");
+        int64_t as = 0, at = 0;
+        for (int64_t k = 0; k < r->n; k++) {
+            const int32_t ls = r->len_s[size_t(k)], lt = r->len_t[size_t(k)];
+            const int32_t m = r->path_len[size_t(k)];
+            if (m < 0) {
+                std::fill(s_to_t + as, s_to_t + as + ls, -1);
+                std::fill(t_to_s + at, t_to_s + at + lt, -1);
+            } else {
+                alignment_from_path(r->paths.data() + 2 * r->path_off[size_t(k)], m, ls, lt, s_to_t + as,
+                                    t_to_s + at);
+            }
+            as += ls;
+            at += lt;
+        }
+    });
 }
 
 int pa_result_alignment(const pa_result *r, int64_t k, int32_t *s_to_t, int32_t *t_to_s) {
