@@ -74,6 +74,9 @@ bool supported_R(int form, int tr, bool fastmax, int R) {
 #ifndef PA_WALK_KT_MAX
 #define PA_WALK_KT_MAX WAVE
 #endif
+#ifndef PA_WALK_ST_MUL
+#define PA_WALK_ST_MUL 1
+#endif
 struct WalkShape {
     int waves;       // wavefronts (problems) per block
     int tile_words;  // traceback words per wavefront tile
@@ -139,7 +142,11 @@ struct TbWalker {
         // step more than R, so tiles hold about R + 1 steps per lane
         KT = L < PA_WALK_KT_MAX ? L : PA_WALK_KT_MAX;
         while (KT > 1 && KT * KT * wpc * (R + 1) > tile_words) KT >>= 1;
+        // steps: what a diagonal path uses crossing the tile's lanes (a
+        // refill costs in proportion to the tile's size; measured against
+        // twice that and against the whole tile)
         ST = tile_words / (KT * wpc);
+        ST = ST < PA_WALK_ST_MUL * (R + 1) * KT ? ST : PA_WALK_ST_MUL * (R + 1) * KT;
         t_st = -1;
         k0 = k1 = 0;
         s0 = s1 = 0;
