@@ -77,6 +77,9 @@ bool supported_R(int form, int tr, bool fastmax, int R) {
 #ifndef PA_WALK_ST_MUL
 #define PA_WALK_ST_MUL 1
 #endif
+#ifndef PA_WALK_LOADS
+#define PA_WALK_LOADS 8
+#endif
 struct WalkShape {
     int waves;       // wavefronts (problems) per block
     int tile_words;  // traceback words per wavefront tile
@@ -203,8 +206,8 @@ struct TbWalker {
             const uint32_t *base = tb + tb_off + (int64_t(t_st) * nsteps + s0) * L * wpc + int64_t(k0) * wpc;
             const int sstride = L * wpc;
             __builtin_amdgcn_wave_barrier();  // the old tile's reads are done
-            // eight loads in flight per lane before their LDS stores
-            constexpr int U = 8;
+            // PA_WALK_LOADS loads in flight per lane before their LDS stores
+            constexpr int U = PA_WALK_LOADS;
             for (int w0 = 0; w0 < cnt; w0 += U * WAVE) {
                 uint32_t v[U];
 #pragma unroll
