@@ -115,7 +115,7 @@ constexpr int SINK_WORDS = 16;
 // R = 0 selects the kernel that serves every strip height in FAST_RS from
 // one queue (the group carries its R).
 #ifndef PA_FAST_RS_LIST
-#define PA_FAST_RS_LIST 8, 12, 16, 20, 24, 28, 32
+#define PA_FAST_RS_LIST 8, 10, 12, 14, 16, 20, 24, 28, 32
 #endif
 constexpr int FAST_RS[] = {PA_FAST_RS_LIST};
 
