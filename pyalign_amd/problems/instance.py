@@ -1,6 +1,7 @@
 """Problem forms and bags (pyalign/problems/instance.py:1-275)."""
 
 import enum
+from operator import attrgetter
 
 import numpy as np
 
@@ -135,18 +136,19 @@ class ProblemBag:
 
     def __init__(self, problems):
         self._problems = problems
+        # list.count compares in C (identity first): every item equal to the first
         self._direction = problems[0].direction
-        if any(p.direction != self._direction for p in problems):
+        if list(map(attrgetter("direction"), problems)).count(self._direction) != len(problems):
             raise ValueError("problems in a bag need to have same direction")
         self._dtype = problems[0].dtype
-        if any(p.dtype != self._dtype for p in problems):
+        if list(map(attrgetter("dtype"), problems)).count(self._dtype) != len(problems):
             raise ValueError("problems in a bag need to have same dtype")
         # a problem's form is fixed by its class: one lookup per class
         reps = {}
         for p in problems:
             reps.setdefault(type(p), p)
         self._form = Form(min(p.form.value for p in reps.values()))
-        shapes = np.array([p.shape for p in problems], dtype=np.int64).reshape(len(problems), 2)
+        shapes = np.array(list(map(attrgetter("shape"), problems)), dtype=np.int64).reshape(len(problems), 2)
         self._shapes = shapes
         self._max_shape = (np.max(shapes[:, 0]), np.max(shapes[:, 1]))
 

@@ -18,6 +18,7 @@ from collections.abc import Sequence
 from functools import cached_property
 
 import numpy as np
+from operator import attrgetter
 
 from . import algorithm
 from .gaps import GapCost, ConstantGapCost
@@ -422,8 +423,8 @@ def _joined_text(problems):
     else UTF-32-LE code points) for encoding on the device, or None when a
     sequence is not a str."""
     try:  # join accepts only str items: lists or arrays of symbols go to the host encoders
-        js = "".join([p.s for p in problems])
-        jt = "".join([p.t for p in problems])
+        js = "".join(map(attrgetter("s"), problems))
+        jt = "".join(map(attrgetter("t"), problems))
     except TypeError:
         return None
     try:
@@ -596,6 +597,8 @@ class Solver:
                 if text is None or "not in alphabet" not in str(e):
                     raise
                 raise _bad_symbol_error(solver, text, payload["off_s"], payload["off_t"]) from None
+            if kind == "score":  # the construct is the identity
+                return list(r)
             return [construct(p, solver, x) for p, x in zip(problems, r)]
 
     def _solve_problem(self, problem):
