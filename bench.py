@@ -440,7 +440,7 @@ def main():
 
     long_pair = fwd_bwd = dtw = api = c1 = None
     if not args.no_extra:
-        c1 = c1_bench(2)
+        c1 = c1_bench(10)
         log(f"global Solution batch: {c1['solve_s']:.3f} s per solve")
         api, api_scores = api_bench(table, ls, lt, codes_s, codes_t)
         api["scores_bitwise_equal"] = bool(np.array_equal(api_scores.view(np.uint32), auto_scores.view(np.uint32)))
