@@ -1,6 +1,7 @@
 """Problem forms and bags (pyalign/problems/instance.py:1-275)."""
 
 import enum
+from itertools import chain
 from operator import attrgetter
 
 import numpy as np
@@ -144,11 +145,11 @@ class ProblemBag:
         if list(map(attrgetter("dtype"), problems)).count(self._dtype) != len(problems):
             raise ValueError("problems in a bag need to have same dtype")
         # a problem's form is fixed by its class: one lookup per class
-        reps = {}
-        for p in problems:
-            reps.setdefault(type(p), p)
-        self._form = Form(min(p.form.value for p in reps.values()))
-        shapes = np.array(list(map(attrgetter("shape"), problems)), dtype=np.int64).reshape(len(problems), 2)
+        classes = set(map(type, problems))
+        reps = [problems[0]] if len(classes) == 1 else [next(p for p in problems if type(p) is c) for c in classes]
+        self._form = Form(min(p.form.value for p in reps))
+        shapes = np.fromiter(chain.from_iterable(map(attrgetter("shape"), problems)), dtype=np.int64,
+                             count=2 * len(problems)).reshape(len(problems), 2)
         self._shapes = shapes
         self._max_shape = (np.max(shapes[:, 0]), np.max(shapes[:, 1]))
 

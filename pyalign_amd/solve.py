@@ -19,7 +19,7 @@ from collections.abc import Sequence
 from functools import cached_property
 
 import numpy as np
-from operator import attrgetter
+from operator import attrgetter, is_, methodcaller
 
 from . import algorithm
 from .gaps import GapCost, ConstantGapCost
@@ -382,16 +382,29 @@ class BinaryMatrixForm(AbstractIndexedMatrixForm):
         return [self._construct(p, self._solver, x) for p, x in zip(batch.problems, r)]
 
 
+def _all_are(values, obj):
+    """Whether every value is obj (identity; the loop runs in C)."""
+    return all(map(is_, values, itertools.repeat(obj)))
+
+
+def _all_have(problems, name, obj):
+    """Whether every problem's attribute name is obj; a missing attribute is False."""
+    try:
+        return _all_are(map(attrgetter(name), problems), obj)
+    except AttributeError:
+        return False
+
+
 def _shared_table(bag):
     sim = bag.problems[0].similarity_lookup_table()
-    if not all(p.similarity_lookup_table() is sim for p in bag.problems):
+    if not _all_are(map(methodcaller("similarity_lookup_table"), bag.problems), sim):
         raise ValueError("similarity table must be identical for all problems in a batch")
     return sim
 
 
 def _shared_eq_ne(bag):
     sim = bag.problems[0].binary_similarity_values()
-    if not all(p.binary_similarity_values() is sim for p in bag.problems):
+    if not _all_are(map(methodcaller("binary_similarity_values"), bag.problems), sim):
         raise ValueError("similarity must be identical for all problems in a batch")
     return sim
 
@@ -406,8 +419,8 @@ def _offsets(lens):
 def _encode_bag(problems, which):
     """Flat codes, offsets and lengths of every problem's s (which=0) or t."""
     enc = getattr(problems[0], "encoder", None)
-    if enc is not None and hasattr(enc, "encode_many") and all(getattr(p, "encoder", None) is enc for p in problems):
-        return enc.encode_many([p.s if which == 0 else p.t for p in problems])
+    if enc is not None and hasattr(enc, "encode_many") and _all_have(problems, "encoder", enc):
+        return enc.encode_many(list(map(attrgetter("s" if which == 0 else "t"), problems)))
     lens = np.array([p.shape[which] for p in problems], dtype=np.int64)
     off = _offsets(lens)
     flat = np.zeros(int(lens.sum()), dtype=np.uint32)
@@ -458,7 +471,7 @@ def _text_payload(problems, ls, lt, indexed):
     lut = None
     enc = getattr(problems[0], "encoder", None)
     if indexed:
-        if not hasattr(enc, "char_lut") or not all(getattr(p, "encoder", None) is enc for p in problems):
+        if not hasattr(enc, "char_lut") or not _all_have(problems, "encoder", enc):
             return None, None
     text = _joined_text(problems)
     if text is None:
