@@ -1954,6 +1954,11 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, int form, hipStream_t
     t.s_scores.ensure(size_t(std::max<int64_t>(t.n, 1)) * 4);
     std::vector<Prob> whole, by_strip;
     for (const Prob &pr : t.ckpt) (words_of(pr) * strips_of(pr) <= budget ? whole : by_strip).push_back(pr);
+    // longest walks first: the walk kernel takes problems in this order, a
+    // wavefront each, more of them than fit the GPU at once
+    std::stable_sort(whole.begin(), whole.end(), [](const Prob &x, const Prob &y) {
+        return int64_t(x.len_s) + x.len_t > int64_t(y.len_s) + y.len_t;
+    });
     auto recompute = [&](const std::vector<Prob> &probs, const std::vector<Group> &groups) {
         upload(st, t.s_probs, probs.data(), probs.size());
         upload(st, t.s_groups, groups.data(), groups.size());
