@@ -29,11 +29,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GCUPS batched Gotoh local align, fwd & fwd+bwd, vs plain GPU path; CPU ref"
 OPS_PER_CELL = 11  # 5 add + 4 max + local zero clamp + running best (Gotoh, local score)
-# MI355X VALU ceiling in lane-operations: the published fp32 vector peak,
-# 157.3 TFLOP/s, counts an FMA as two operations; a SIMD issues at most two
-# wave64 VALU instructions per four cycles (SQ_ACTIVE_INST_VALU2 counts the
-# dual issues), i.e. 157.3 / 2 = 78.65 T single-operation lane instructions/s
-VALU_PEAK_TOPS = 157.3 / 2
+# MI355X's published fp32 vector peak, 157.3 TFLOP/s; each algorithmic add
+# or max counts as one operation against it
+VALU_PEAK_TFLOPS = 157.3
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -431,6 +440,7 @@ def main():
         gcups, threads, secs, cpu_scores = cpu_baseline(table, ls, lt, codes_s, codes_t, off_s, off_t, sample, lim)
         g1, _, secs1, _ = cpu_baseline(table, ls, lt, codes_s, codes_t, off_s, off_t, min(2048, n), lim, threads=1)
         cpu = {"value": round(gcups, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
                "single_core": {"value": round(g1, 4), "sample": f"first {min(2048, n)} pairs ({secs1:.1f} s)"},
                "sample": (f"all {n} pairs" if sample == n else f"first {sample} of the {n} pairs")
                          + f" ({secs:.1f} s; oracle's 16-lane batched restatement, Morton batches of 16 as the "
@@ -460,10 +470,10 @@ def main():
         "config": {"workload": "batched local Gotoh score-only: 65536 pairs, lengths log-uniform [50,2000], "
                                "20-letter integer table, affine open 10 extend 1",
                    "pairs": n, "cells": cells, "forward": "score", "launches_per_step": launches},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 3),
-                     "unit": "Tops/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": traffic(),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK_TFLOPS, 4), "traffic": traffic(),
                      "ops_per_cell": OPS_PER_CELL, "kernel_ms_per_step": round(kms, 3),
-                     "peak_basis": "fp32 157.3 TFLOP/s / 2: one add or max per lane op (an FMA counts 2)"},
+                     "peak_basis": "MI355X published fp32 vector peak; one add or max = one operation"},
         "plain_gpu": plain,
         "cpu_baseline": cpu,
         "fwd_bwd": fwd_bwd,

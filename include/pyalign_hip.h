@@ -100,6 +100,17 @@ void pa_solver_free(pa_solver *solver);
 int pa_solver_set_gap_costs(pa_solver *solver, const float *w_s, int64_t n_s, const float *w_t,
                             int64_t n_t);
 
+/* Linear or affine gap costs changed in place (no reference counterpart:
+ * a reference solver's costs are fixed at creation, factory.h:33-64).
+ * gap_s / gap_t hold the to_special_case() numbers as in pa_options_desc
+ * (linear {g, -}, affine {u = open - extend, v = extend}); the solver keeps
+ * its limits and device buffers and recomputes its border vectors exactly as
+ * pa_create_solver does.  For differentiable training loops whose gap costs
+ * are parameters: one solver serves every step.  Work already queued on the
+ * solver's stream sees the old costs, later solves the new ones; a tape keeps
+ * the costs of its forward. */
+int pa_solver_set_gap_params(pa_solver *solver, const double *gap_s, const double *gap_t);
+
 /* Solver.batch_size (pyalign.h:24, factory.h:422): lanes per call. */
 int pa_solver_batch_size(const pa_solver *solver);
 
@@ -126,8 +137,14 @@ int pa_solve_binary(pa_solver *solver, const uint32_t *a, int64_t La, const uint
  * Batching is result-neutral (each lane's cells depend only on its own
  * region, solver.h:2593-2705), so results equal the per-batch calls.
  *
- * Indexed form: codes_s/codes_t are uint8 symbol codes (< K <= 255),
- *   sequence k occupies codes_s[off_s[k] .. off_s[k]+len_s[k]).
+ * Indexed form: codes_s/codes_t are symbol codes, uint8 (code_bytes 0 or 1)
+ *   or uint16 (code_bytes 2); sequence k occupies
+ *   codes_s[off_s[k] .. off_s[k]+len_s[k]).  The table is float32[K][K_t]
+ *   (K_t = 0: square), s codes index its rows (< K), t codes its columns
+ *   (< K_t), as indexed_matrix_form checks them (factory.h:602-614).  Any
+ *   size up to 32,767 symbols per side: tables of up to 128 symbols are
+ *   staged in LDS, larger ones are read from global memory (L2-resident)
+ *   with uint16 codes (host uint8 codes are widened on the device).
  * Binary form: codes are uint32 values compared for equality.
  * Dense form: dense + dense_off[k] is problem k's row-major [len_s][len_t].
  * Series form (DTW): codes_s/codes_t are float32 series of `dim` columns,
@@ -145,8 +162,8 @@ typedef struct pa_ragged {
     const int64_t *off_t;
     const int32_t *len_s;
     const int32_t *len_t;
-    const float *table; /* indexed: float32[K][K] (host) */
-    int32_t K;
+    const float *table; /* indexed: float32[K][K_t] (host) */
+    int32_t K;          /* indexed: table rows (s symbols) */
     int32_t dim;        /* series dimension */
     float eq, ne;       /* binary */
     const float *dense; /* dense payload */
@@ -163,6 +180,8 @@ typedef struct pa_ragged {
     int32_t raw_width;
     int32_t lut_len;
     const uint8_t *lut;
+    int32_t K_t;        /* indexed: table columns (t symbols); 0 = K */
+    int32_t code_bytes; /* indexed codes given (raw_width 0): 1 = uint8 (0 too), 2 = uint16 */
 } pa_ragged;
 
 /* Scheduling of a ragged batch. */

@@ -39,6 +39,7 @@ class Ragged(ctypes.Structure):
         ("dense", ctypes.c_void_p), ("dense_off", ctypes.c_void_p),
         ("dense_ld", ctypes.c_int64),
         ("raw_width", ctypes.c_int32), ("lut_len", ctypes.c_int32), ("lut", ctypes.c_void_p),
+        ("K_t", ctypes.c_int32), ("code_bytes", ctypes.c_int32),
     ]
 
 
@@ -67,6 +68,7 @@ SIGNATURES = {
     "pa_solver_set_checkpoint": (ctypes.c_int, [_P, _I64]),
     "pa_solver_set_stream": (ctypes.c_int, [_P, _P, _I32]),
     "pa_solver_set_gap_costs": (ctypes.c_int, [_P, _P, _I64, _P, _I64]),
+    "pa_solver_set_gap_params": (ctypes.c_int, [_P, _P, _P]),
     "pa_solver_last_bad_symbol": (ctypes.c_int, [_P, _P, _P]),
     "pa_solver_last_schedule": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "pa_forward_ragged": (ctypes.c_int, [_P, ctypes.POINTER(Ragged), _P, ctypes.POINTER(_P)]),

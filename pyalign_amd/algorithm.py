@@ -468,15 +468,12 @@ class Solver:
         check(lib().pa_solve_indexed(self._h, a.ctypes.data, a.shape[1], b.ctypes.data, b.shape[1], a.shape[0],
                                      sim.ctypes.data, sim.shape[0], sim.shape[1], length.ctypes.data,
                                      ctypes.byref(h)))
-        table = None
-        if kind.startswith("solution"):
-            K = max(sim.shape)  # square table; codes stay below each side's size
-            table = np.zeros((K, K), dtype=np.float32)
-            table[:sim.shape[0], :sim.shape[1]] = sim
+        table = sim.copy() if kind.startswith("solution") else None
+        code_t = np.uint16 if max(sim.shape) > 128 else np.uint8
 
         def one(k):
-            return "indexed", dict(codes_s=a[k, :length[0, k]].astype(np.uint8),
-                                   codes_t=b[k, :length[1, k]].astype(np.uint8), off_s=[0], off_t=[0], table=table)
+            return "indexed", dict(codes_s=a[k, :length[0, k]].astype(code_t),
+                                   codes_t=b[k, :length[1, k]].astype(code_t), off_s=[0], off_t=[0], table=table)
 
         return self._wrap(h, length[0], length[1], kind, one)
 
@@ -570,6 +567,22 @@ class Solver:
         ext = stream is not None
         check(lib().pa_solver_set_stream(self._h, ctypes.c_void_p(int(stream) if ext else 0), int(ext)))
 
+    def set_gap_cost(self, gap_cost):
+        """Replace the solver's linear or affine gap cost (a GapCost, or a dict
+        {'s': ..., 't': ...}) in place: same limits, buffers and plan cache,
+        new border vectors.  The kind must stay the same."""
+        if isinstance(gap_cost, dict):
+            gs, gt = gap_cost.get("s"), gap_cost.get("t")
+        else:
+            gs = gt = gap_cost
+        (ks, vs), (kt, vt) = _gap_spec(gs), _gap_spec(gt)
+        kind = {_lib.PA_GAP_NONE: "linear", _lib.PA_GAP_LINEAR: "linear", _lib.PA_GAP_AFFINE: "affine"}
+        if kind.get(ks) != self._opts.gap_kind or kind.get(kt) != self._opts.gap_kind:
+            raise ValueError("the gap cost kind of a solver cannot change")
+        ds = (ctypes.c_double * 2)(*vs)
+        dt = (ctypes.c_double * 2)(*vt)
+        check(lib().pa_solver_set_gap_params(self._h, ds, dt))
+
     def set_graphs(self, enable):
         """Replay repeated launch sequences from a captured hipGraph (default on)."""
         check(lib().pa_solver_set_graphs(self._h, int(bool(enable))))
@@ -661,10 +674,13 @@ _FORMS = {"dense": _lib.PA_FORM_DENSE, "indexed": _lib.PA_FORM_INDEXED,
 
 def make_ragged(form, len_s, len_t, codes_s=None, codes_t=None, off_s=None, off_t=None, table=None,
                 eq=0.0, ne=0.0, dense=None, dense_off=None, dense_ld=0, dim=0, on_device=False, raw_width=0,
-                lut=None):
+                lut=None, code_bytes=0):
     """Build a pa_ragged descriptor; returns (struct, arrays to keep alive).
 
     With on_device=True the payload arguments are device pointers (ints).
+    Indexed form: table is K x K_t (rows: s symbols, columns: t symbols);
+    code_bytes 1 or 2 fixes the code width (0: uint8 up to 128 symbols,
+    uint16 beyond).
     raw_width 1 or 4: codes_s/codes_t are raw symbols (bytes of latin-1 or
     UTF-32-LE text) that the device encodes, through lut for the indexed
     form.
@@ -702,13 +718,20 @@ def make_ragged(form, len_s, len_t, codes_s=None, codes_t=None, off_s=None, off_
             rg.lut = arr(lut, np.uint8)
             rg.lut_len = len(lut)
     if form == "indexed":
-        if not raw_width:
-            rg.codes_s = payload(codes_s, np.uint8)
-            rg.codes_t = payload(codes_t, np.uint8)
         t = np.ascontiguousarray(table, dtype=np.float32)
+        if t.ndim != 2:
+            raise ValueError(f"similarity table has shape {t.shape}, expected two dimensions")
         keep.append(t)
         rg.table = t.ctypes.data
-        rg.K = t.shape[0]
+        rg.K, rg.K_t = t.shape
+        if not raw_width:
+            # uint8 codes for tables of up to 128 symbols (staged in LDS),
+            # uint16 beyond (read from global memory); device-resident codes
+            # are taken in the width the caller gives
+            wide = code_bytes == 2 if code_bytes else max(t.shape) > 128
+            rg.code_bytes = 2 if wide else 1
+            rg.codes_s = payload(codes_s, np.uint16 if wide else np.uint8)
+            rg.codes_t = payload(codes_t, np.uint16 if wide else np.uint8)
     elif form == "binary":
         if not raw_width:
             rg.codes_s = payload(codes_s, np.uint32)

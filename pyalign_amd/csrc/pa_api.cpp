@@ -186,6 +186,12 @@ struct pa_solver {
     DevBuf b_raw_s, b_raw_t, b_lut, b_bad;
     DevBuf b_safe, b_sink;  // KArgs::safe, KArgs::sink
     DevBuf b_ckpt_tb;       // checkpointed backward: recomputed tracebacks (scratch)
+    DevBuf b_tape_borders;  // a checkpointed tape's copy of the border vectors (recycled)
+    // pinned staging of border uploads after a gap change (pa_solver_set_gap_params)
+    float *border_stage = nullptr;
+    size_t border_stage_cap = 0;
+    hipEvent_t border_ev = nullptr;
+    bool border_pending = false;
     // checkpointed backward: scratch words per chunk when set (> 0), else
     // sized from the free device memory
     int64_t ckpt_words = std::getenv("PYALIGN_AMD_CKPT_SCRATCH_WORDS")
@@ -281,6 +287,7 @@ struct pa_tape {
     // checkpointed problems: their strip boundary rows, the forward's kernel
     // arguments (to recompute a strip) and their descriptors
     DevBuf rowbuf;
+    DevBuf borders;  // the forward's border vectors (the solver's may change before the backward)
     pa::KArgs kargs{};
     std::vector<pa::Prob> ckpt;
     int64_t band_slots = 0;
@@ -295,6 +302,9 @@ pa_solver::~pa_solver() {
     if (stage_pending) (void)hipEventSynchronize(stage_ev);
     if (stage) (void)hipHostFree(stage);
     if (stage_ev) (void)hipEventDestroy(stage_ev);
+    if (border_pending) (void)hipEventSynchronize(border_ev);
+    if (border_stage) (void)hipHostFree(border_stage);
+    if (border_ev) (void)hipEventDestroy(border_ev);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -469,7 +479,7 @@ struct Job {
 // wavefront sweeps its strips of BAND_L x BAND_R rows in order and keeps
 // every strip's bottom row, from which the backward recomputes it.
 Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, int simds,
-               const uint8_t *ckpt = nullptr) {
+               const uint8_t *ckpt = nullptr, bool allow_band = true) {
     std::vector<int> Rs;
     if (job.fast) {
         Rs.assign(std::begin(FAST_RS), std::end(FAST_RS));
@@ -511,7 +521,7 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         double band;  // > 0: band mode, its serial time
         bool ckpt;    // a checkpointed problem (strip rows kept, no traceback)
     };
-    const bool band_ok = job.tr != TR_ALL && supported_R(job.form, job.tr, false, BAND_R);
+    const bool band_ok = allow_band && job.tr != TR_ALL && supported_R(job.form, job.tr, false, BAND_R);
     std::vector<Item> items(static_cast<size_t>(n));
     std::vector<Item> bands, ckpts;
     for (int64_t k = 0; k < n; k++) {
@@ -735,11 +745,18 @@ struct Input {
     bool matrices = false;  // keep every layer's cell values (Solution matrices)
     // raw symbols to encode on the device: 1 or 4 bytes each (0: codes given)
     int raw_width = 0;
-    const uint8_t *lut = nullptr;  // indexed form: symbol -> code, 255 = none
+    const uint8_t *lut = nullptr;  // indexed form: symbol -> code, 255 = none (s side)
     int32_t lut_len = 0;
+    const uint8_t *lut_t = nullptr;  // t side's table (null: the s side's)
+    int32_t lut_t_len = 0;
     int64_t syms_s = 0, syms_t = 0;  // symbols per side
-    std::vector<uint8_t> lut_store;  // an identity table that checks host codes on the device
+    // identity tables that check host uint8 codes on the device (s codes
+    // below the table's rows, t codes below its columns)
+    std::vector<uint8_t> lut_store, lut_t_store;
     const char *bad_s = "item not in alphabet", *bad_t = "item not in alphabet";
+    // indexed form: the table is read from global memory with uint16 codes
+    // (F_TABLE_G) instead of staged in LDS with uint8 codes
+    bool table_g = false;
 };
 
 int goal_to_tr(int goal) { return goal == PA_GOAL_SCORE ? TR_SCORE : (goal == PA_GOAL_ONE ? TR_ONE : TR_ALL); }
@@ -782,19 +799,26 @@ void stage_payload(pa_solver &s, const Input &in, hipStream_t st, const void *&c
         raw_t = s.b_raw_t.p;
     }
     const bool indexed = in.form == PA_FORM_INDEXED;
-    const size_t cw = indexed ? 1 : 4;  // code width
+    const int cw = indexed ? (in.table_g ? 2 : 1) : 4;  // code width
     s.b_cs.ensure(size_t(std::max<int64_t>(in.syms_s, 1)) * cw);
     s.b_ct.ensure(size_t(std::max<int64_t>(in.syms_t, 1)) * cw);
-    const uint8_t *lut = nullptr;
+    const uint8_t *lut = nullptr, *lut_t = nullptr;
+    int32_t lut_t_len = in.lut_len;
     if (indexed) {
-        upload(st, s.b_lut, in.lut, size_t(in.lut_len));
+        const size_t at = (size_t(in.lut_len) + 255) / 256 * 256;
+        lut_t_len = in.lut_t ? in.lut_t_len : in.lut_len;
+        s.b_lut.ensure(at + size_t(std::max(lut_t_len, 1)));
+        HIPCHK(hipMemcpyAsync(s.b_lut.p, in.lut, size_t(in.lut_len), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(static_cast<char *>(s.b_lut.p) + at, in.lut_t ? in.lut_t : in.lut, size_t(lut_t_len),
+                              hipMemcpyHostToDevice, st));
         lut = s.b_lut.as<uint8_t>();
+        lut_t = lut + at;
     }
     s.b_bad.ensure(16);
     unsigned long long *bad = s.b_bad.as<unsigned long long>();
     HIPCHK(launch_fill_u32(s.b_bad.as<uint32_t>(), 4, 0xffffffffu, st));
-    HIPCHK(launch_translate(raw_s, in.raw_width, in.syms_s, lut, in.lut_len, s.b_cs.p, bad, s.cus, st));
-    HIPCHK(launch_translate(raw_t, in.raw_width, in.syms_t, lut, in.lut_len, s.b_ct.p, bad + 1, s.cus, st));
+    HIPCHK(launch_translate(raw_s, in.raw_width, in.syms_s, lut, in.lut_len, s.b_cs.p, cw, bad, s.cus, st));
+    HIPCHK(launch_translate(raw_t, in.raw_width, in.syms_t, lut_t, lut_t_len, s.b_ct.p, cw, bad + 1, s.cus, st));
     codes_s = s.b_cs.p;
     codes_t = s.b_ct.p;
     if (indexed) {
@@ -847,7 +871,7 @@ void run_wsb(pa_solver &s, const Input &in, const Job &job, std::unique_ptr<pa_r
     const void *codes_s = nullptr, *codes_t = nullptr;
     const float *dense = nullptr;
     stage_payload(s, in, st, codes_s, codes_t, dense);
-    if (job.form == F_TABLE) upload(st, s.b_table, in.table.data(), in.table.size());
+    if (job.form == F_TABLE || job.form == F_TABLE_G) upload(st, s.b_table, in.table.data(), in.table.size());
     s.b_dvals.ensure(size_t(std::max<int64_t>(words, 1)) * 4);
     float *scores = scores_device;
     if (!scores) {
@@ -959,6 +983,105 @@ void run_wsb(pa_solver &s, const Input &in, const Job &job, std::unique_ptr<pa_r
     *result = std::move(r);
 }
 
+// A plan's problems get their lengths and offsets: payload, traceback
+// words, strip-boundary row buffer and cell values.
+void place_plan(Plan &pl, const Input &in, int bits, bool keep_vals, int val_layers) {
+    for (Prob &pr : pl.probs) {
+        const int64_t k = pr.out;
+        pr.len_s = in.len_s[k];
+        pr.len_t = in.len_t[k];
+        if (in.form == PA_FORM_DENSE) {
+            pr.off_s = in.dense_off ? in.dense_off[k] : k;
+            pr.off_t = 0;
+            pr.si = in.dense_si_fixed ? in.dense_si_fixed : pr.len_t;
+            pr.sj = in.dense_sj_fixed ? in.dense_sj_fixed : 1;
+        } else {
+            pr.off_s = in.off_s[k];
+            pr.off_t = in.off_t[k];
+        }
+        const int64_t rps = int64_t(pr.L) * pr.R;
+        const int64_t nst = (pr.len_s > 0 && pr.len_t > 0) ? (pr.len_s + rps - 1) / rps : 0;
+        if (bits && pr.tb_off != NO_TB_OFF) {
+            pr.tb_off = pl.tb_words;
+            pl.tb_words += nst * (pr.len_t + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
+        }
+        if (nst > 1) {
+            pr.rb_off = pl.rowbuf;
+            pl.rowbuf += pr.band >= 0 ? (nst - 1) * int64_t(pr.len_t) : int64_t(pr.len_t);
+        }
+        if (keep_vals) {
+            pr.dv_off = pl.dvals;
+            pl.dvals += int64_t(val_layers) * pr.len_s * pr.len_t;
+        }
+        pl.cells += double(pr.len_s) * double(pr.len_t);
+    }
+}
+
+// The DP launches of a plan: one spec (kernel, strip height, grid, LDS) and
+// argument block per launch.  `a` holds the buffers; queue_base the launches'
+// ticket counters.
+void build_launches(pa_solver &s, const Job &job, const Input &in, const Plan &plan, const KArgs &a,
+                    uint32_t *queue_base, std::vector<LaunchSpec> &specs, std::vector<KArgs> &largs) {
+    // dense rows with unit column stride and >= DROWS_MIN_LT columns in every
+    // non-empty problem: the kernel stages each lane's rows through LDS (F_DROWS)
+    const int64_t n = in.n;
+    bool drows = job.form == F_DENSE && (in.dense_sj_fixed == 0 || in.dense_sj_fixed == 1) && !s.no_drows;
+    for (int64_t k = 0; k < n && drows; k++)
+        if (in.len_s[k] > 0 && in.len_t[k] > 0 && in.len_t[k] < DROWS_MIN_LT) drows = false;
+    for (size_t li = 0; li < plan.launches.size(); li++) {
+        int R = plan.launches[li].first;
+        bool fastmax = job.fast;
+        if (R >= 1000) {  // band-mode strips of a fast job: the general kernel
+            R -= 1000;
+            fastmax = false;
+        }
+        const int g0 = plan.launches[li].second.first, g1 = plan.launches[li].second.second;
+        if (g1 <= g0) continue;
+        LaunchSpec spec;
+        std::memset(&spec, 0, sizeof(spec));
+        spec.loc = job.loc;
+        spec.dir = job.dir;
+        spec.gap = job.gap;
+        spec.form = job.form;
+        spec.tr = job.tr;
+        spec.R = R;
+        spec.fastmax = fastmax;
+        spec.lds_bytes = job.form == F_TABLE ? size_t(in.K1) * in.K1 * 4 : 0;
+        if (drows && R == DROWS_WIN) {
+            spec.form = F_DROWS;
+            spec.lds_bytes = DROWS_LDS;
+        }
+        if (fastmax && R == 0 && in.K1 >= 2) {
+            const size_t pair_bytes = size_t(in.K1) * in.K1 * (in.K1 - 1) * 8;
+            if (pair_bytes <= PAIR_LDS_MAX && !s.no_pair) {
+                spec.pair = 1;
+                spec.lds_bytes = pair_bytes;
+            }
+        }
+        const std::vector<int> okey = {spec.loc, spec.dir, spec.gap, spec.form, spec.tr, spec.R,
+                                       int(spec.fastmax), int(spec.lds_bytes), spec.pair, in.dim};
+        int waves_per_cu = 0;
+        for (auto &e : s.occ_cache)
+            if (e.first == okey) waves_per_cu = e.second;
+        if (!waves_per_cu) {
+            waves_per_cu = std::max(1, occupancy_waves(spec, in.dim));
+            s.occ_cache.push_back({okey, waves_per_cu});
+        }
+        // the fast kernel's large workgroups only where the groups fill every
+        // CU with them; fewer groups spread over more, smaller workgroups
+        spec.block = fastmax && R == 0 && int64_t(g1 - g0) >= int64_t(s.cus) * (FAST_BLOCK / WAVE) ? FAST_BLOCK : 256;
+        const int wpb = spec.block / WAVE;  // waves per workgroup
+        const int64_t wg_needed = (int64_t(g1 - g0) + wpb - 1) / wpb;
+        spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg_needed, int64_t(s.cus) * std::max(1, waves_per_cu / wpb))));
+        KArgs la = a;
+        la.groups = a.groups + g0;
+        la.n_groups = g1 - g0;
+        la.queue = queue_base + li;
+        specs.push_back(spec);
+        largs.push_back(la);
+    }
+}
+
 void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, float *scores_device,
          pa_tape *tape = nullptr) {
     ensure_device(s);
@@ -984,7 +1107,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     job.tr = goal_to_tr(o.goal);
     switch (in.form) {
         case PA_FORM_DENSE: job.form = F_DENSE; break;
-        case PA_FORM_INDEXED: job.form = F_TABLE; break;
+        case PA_FORM_INDEXED: job.form = in.table_g ? F_TABLE_G : F_TABLE; break;
         case PA_FORM_BINARY: job.form = F_BINARY; break;
         case PA_FORM_SERIES: job.form = F_SERIES; break;
         default: fail(PA_EINVAL, "unknown problem form");
@@ -1045,37 +1168,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         pc.plan = in.schedule == PA_SCHED_PLAIN ? plan_plain(job, in.len_s, in.len_t, n)
                                                 : plan_auto(job, in.len_s, in.len_t, n, s.cus * 4,
                                                             n_ckpt ? ckpt.data() : nullptr);
-        Plan &pl = pc.plan;
-        // offsets: payload, traceback, row buffer, D values
-        for (Prob &pr : pl.probs) {
-            const int64_t k = pr.out;
-            pr.len_s = in.len_s[k];
-            pr.len_t = in.len_t[k];
-            if (in.form == PA_FORM_DENSE) {
-                pr.off_s = in.dense_off ? in.dense_off[k] : k;
-                pr.off_t = 0;
-                pr.si = in.dense_si_fixed ? in.dense_si_fixed : pr.len_t;
-                pr.sj = in.dense_sj_fixed ? in.dense_sj_fixed : 1;
-            } else {
-                pr.off_s = in.off_s[k];
-                pr.off_t = in.off_t[k];
-            }
-            const int64_t rps = int64_t(pr.L) * pr.R;
-            const int64_t nst = (pr.len_s > 0 && pr.len_t > 0) ? (pr.len_s + rps - 1) / rps : 0;
-            if (bits && pr.tb_off != NO_TB_OFF) {
-                pr.tb_off = pl.tb_words;
-                pl.tb_words += nst * (pr.len_t + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
-            }
-            if (nst > 1) {
-                pr.rb_off = pl.rowbuf;
-                pl.rowbuf += pr.band >= 0 ? (nst - 1) * int64_t(pr.len_t) : int64_t(pr.len_t);
-            }
-            if (keep_vals) {
-                pr.dv_off = pl.dvals;
-                pl.dvals += int64_t(val_layers) * pr.len_s * pr.len_t;
-            }
-            pl.cells += double(pr.len_s) * double(pr.len_t);
-        }
+        place_plan(pc.plan, in, bits, keep_vals, val_layers);
         pc.uploaded = false;
         pc.key = key;
         pc.len_s.assign(in.len_s, in.len_s + n);
@@ -1121,7 +1214,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     const void *codes_s = nullptr, *codes_t = nullptr;
     const float *dense = nullptr;
     stage_payload(s, in, st, codes_s, codes_t, dense);
-    if (job.form == F_TABLE) upload(st, s.b_table, in.table.data(), in.table.size());
+    if (job.form == F_TABLE || job.form == F_TABLE_G) upload(st, s.b_table, in.table.data(), in.table.size());
     s.b_rowbuf.ensure(size_t(std::max<int64_t>(plan.rowbuf, 1)) * sizeof(float2));
     float *scores = scores_device;
     if (!scores) {
@@ -1175,63 +1268,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     // ---- launches: specs first (occupancy queries stay outside any capture)
     std::vector<LaunchSpec> specs;
     std::vector<KArgs> largs;
-    // dense rows with unit column stride and >= DROWS_MIN_LT columns in every
-    // non-empty problem: the kernel stages each lane's rows through LDS (F_DROWS)
-    bool drows = job.form == F_DENSE && (in.dense_sj_fixed == 0 || in.dense_sj_fixed == 1) && !s.no_drows;
-    for (int64_t k = 0; k < n && drows; k++)
-        if (in.len_s[k] > 0 && in.len_t[k] > 0 && in.len_t[k] < DROWS_MIN_LT) drows = false;
-    for (size_t li = 0; li < plan.launches.size(); li++) {
-        int R = plan.launches[li].first;
-        bool fastmax = job.fast;
-        if (R >= 1000) {  // band-mode strips of a fast job: the general kernel
-            R -= 1000;
-            fastmax = false;
-        }
-        const int g0 = plan.launches[li].second.first, g1 = plan.launches[li].second.second;
-        if (g1 <= g0) continue;
-        LaunchSpec spec;
-        std::memset(&spec, 0, sizeof(spec));
-        spec.loc = job.loc;
-        spec.dir = job.dir;
-        spec.gap = job.gap;
-        spec.form = job.form;
-        spec.tr = job.tr;
-        spec.R = R;
-        spec.fastmax = fastmax;
-        spec.lds_bytes = job.form == F_TABLE ? size_t(in.K1) * in.K1 * 4 : 0;
-        if (drows && R == DROWS_WIN) {
-            spec.form = F_DROWS;
-            spec.lds_bytes = DROWS_LDS;
-        }
-        if (fastmax && R == 0 && in.K1 >= 2) {
-            const size_t pair_bytes = size_t(in.K1) * in.K1 * (in.K1 - 1) * 8;
-            if (pair_bytes <= PAIR_LDS_MAX && !s.no_pair) {
-                spec.pair = 1;
-                spec.lds_bytes = pair_bytes;
-            }
-        }
-        const std::vector<int> okey = {spec.loc, spec.dir, spec.gap, spec.form, spec.tr, spec.R,
-                                       int(spec.fastmax), int(spec.lds_bytes), spec.pair, in.dim};
-        int waves_per_cu = 0;
-        for (auto &e : s.occ_cache)
-            if (e.first == okey) waves_per_cu = e.second;
-        if (!waves_per_cu) {
-            waves_per_cu = std::max(1, occupancy_waves(spec, in.dim));
-            s.occ_cache.push_back({okey, waves_per_cu});
-        }
-        // the fast kernel's large workgroups only where the groups fill every
-        // CU with them; fewer groups spread over more, smaller workgroups
-        spec.block = fastmax && R == 0 && int64_t(g1 - g0) >= int64_t(s.cus) * (FAST_BLOCK / WAVE) ? FAST_BLOCK : 256;
-        const int wpb = spec.block / WAVE;  // waves per workgroup
-        const int64_t wg_needed = (int64_t(g1 - g0) + wpb - 1) / wpb;
-        spec.grid = int(std::max<int64_t>(1, std::min<int64_t>(wg_needed, int64_t(s.cus) * std::max(1, waves_per_cu / wpb))));
-        KArgs la = a;
-        la.groups = a.groups + g0;
-        la.n_groups = g1 - g0;
-        la.queue = s.b_queue.as<uint32_t>() + li;
-        specs.push_back(spec);
-        largs.push_back(la);
-    }
+    build_launches(s, job, in, plan, a, s.b_queue.as<uint32_t>(), specs, largs);
     const int n_band = int(plan.band_probs.size());
     // the whole sequence: queue tickets and band counters cleared by kernels
     // (no memset nodes), the DP launches, the band fold
@@ -1331,6 +1368,19 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             std::swap(tape->rowbuf.cap, s.b_rowbuf.cap);
             tape->kargs = a;  // a.rowbuf now points into the tape's row buffer
             tape->band_slots = plan.band_slots;
+            // the strip recomputation reads the borders of this forward's gap
+            // costs: the tape keeps a copy (a recycled buffer), since the
+            // solver's can be changed by pa_solver_set_gap_params before the
+            // backward runs
+            const size_t ns = s.border_s.size(), nt = s.border_t.size();
+            std::swap(tape->borders.p, s.b_tape_borders.p);
+            std::swap(tape->borders.cap, s.b_tape_borders.cap);
+            tape->borders.ensure((ns + nt) * 4);
+            HIPCHK(hipMemcpyAsync(tape->borders.p, s.b_border_s.p, ns * 4, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(tape->borders.as<float>() + ns, s.b_border_t.p, nt * 4, hipMemcpyDeviceToDevice,
+                                  st));
+            tape->kargs.border_s = tape->borders.as<float>();
+            tape->kargs.border_t = tape->borders.as<float>() + ns;
         }
         s.last_plan.uploaded = false;  // its descriptors moved to the tape
         return;
@@ -1587,6 +1637,13 @@ int pa_solve_dense(pa_solver *s, const float *sim, int64_t Ls, int64_t Lt, int64
     });
 }
 
+// Table sizes the indexed form serves: 1 .. 32,767 symbols per side (codes
+// times the padded table stride stay within int32 in the kernels).
+static void check_alphabet(int64_t K0, int64_t K1) {
+    if (K0 < 1 || K1 < 1) fail(PA_EINVAL, "the similarity table needs at least one symbol per side");
+    if (K0 > 32767 || K1 > 32767) fail(PA_EINVAL, "alphabets above 32,767 symbols are not supported");
+}
+
 static void indexed_input(Input &in, const uint32_t *a, int64_t La, const uint32_t *b, int64_t Lb, int64_t B,
                           const float *sim, int64_t K0, int64_t K1, int dir, std::vector<uint8_t> &cs,
                           std::vector<uint8_t> &ct, std::vector<int64_t> &os, std::vector<int64_t> &ot) {
@@ -1596,12 +1653,21 @@ static void indexed_input(Input &in, const uint32_t *a, int64_t La, const uint32
     // indexed_matrix_form::check (factory.h:602-614)
     if (B * La > 0 && amax >= uint64_t(K0)) fail(PA_EINVAL, "out of bounds index in a");
     if (B * Lb > 0 && bmax >= uint64_t(K1)) fail(PA_EINVAL, "out of bounds index in b");
-    if (std::max(K0, K1) + 1 > MAX_LDS_TABLE)
-        fail(PA_EINVAL, "alphabets above 128 symbols are not supported by the MI355X indexed form");
-    cs.resize(size_t(B * La) + 1);
-    ct.resize(size_t(B * Lb) + 1);
-    for (int64_t k = 0; k < B * La; k++) cs[size_t(k)] = uint8_t(a[k]);
-    for (int64_t k = 0; k < B * Lb; k++) ct[size_t(k)] = uint8_t(b[k]);
+    check_alphabet(K0, K1);
+    // up to 128 symbols: uint8 codes, table staged in LDS; beyond: uint16
+    // codes, table read from global memory
+    in.table_g = std::max(K0, K1) + 1 > MAX_LDS_TABLE;
+    const size_t cw = in.table_g ? 2 : 1;
+    cs.resize((size_t(B * La) + 1) * cw);
+    ct.resize((size_t(B * Lb) + 1) * cw);
+    for (int64_t k = 0; k < B * La; k++) {
+        if (cw == 2) reinterpret_cast<uint16_t *>(cs.data())[k] = uint16_t(a[k]);
+        else cs[size_t(k)] = uint8_t(a[k]);
+    }
+    for (int64_t k = 0; k < B * Lb; k++) {
+        if (cw == 2) reinterpret_cast<uint16_t *>(ct.data())[k] = uint16_t(b[k]);
+        else ct[size_t(k)] = uint8_t(b[k]);
+    }
     os.resize(static_cast<size_t>(B));
     ot.resize(static_cast<size_t>(B));
     for (int64_t k = 0; k < B; k++) {
@@ -1694,36 +1760,68 @@ static void ragged_input(const pa_solver *s, const pa_ragged *b, Input &in) {
     }
     switch (b->form) {
         case PA_FORM_INDEXED: {
-            if (b->K < 1 || b->K + 1 > MAX_LDS_TABLE) fail(PA_EINVAL, "alphabet size must be 1..128");
+            const int64_t K0 = b->K, K1 = b->K_t > 0 ? b->K_t : b->K;  // table rows, columns
+            check_alphabet(K0, K1);
             if (!b->table) fail(PA_EINVAL, "null table");
-            in.bytes_s = end_s;
-            in.bytes_t = end_t;
+            const int cb = b->code_bytes == 0 ? 1 : b->code_bytes;
+            if (cb != 1 && cb != 2) fail(PA_EINVAL, "indexed codes are 1 or 2 bytes wide");
+            // up to 128 symbols and uint8 codes: the table is staged in LDS;
+            // otherwise read from global memory with uint16 codes
+            in.table_g = std::max(K0, K1) + 1 > MAX_LDS_TABLE || cb == 2;
             if (b->raw_width) {
                 // codes come from the device translation; its table must map
                 // into the alphabet
                 if (!b->lut || b->lut_len < 1) fail(PA_EINVAL, "raw symbols need a code table");
                 for (int32_t q = 0; q < b->lut_len; q++)
-                    if (b->lut[q] != 255 && b->lut[q] >= b->K) fail(PA_EINVAL, "code table entry beyond the alphabet");
+                    if (b->lut[q] != 255 && (b->lut[q] >= K0 || b->lut[q] >= K1))
+                        fail(PA_EINVAL, "code table entry beyond the alphabet");
                 in.lut = b->lut;
                 in.lut_len = b->lut_len;
-            } else if (!b->on_device) {
-                // every code indexes the table staged in LDS: host codes go
-                // through the device encoder with an identity table, which
-                // flags codes >= K before any DP launch
+            } else if (b->on_device) {
+                if (in.table_g && cb != 2)
+                    fail(PA_EINVAL, "device-resident codes for a table above 128 symbols must be uint16 (code_bytes 2)");
+                in.bytes_s = end_s * cb;
+                in.bytes_t = end_t * cb;
+            } else if (cb == 1) {
+                // host uint8 codes go through the device encoder with identity
+                // tables (s codes below the table's rows, t codes below its
+                // columns), which flags an out-of-range code before any DP
+                // launch and widens the codes where the table stays in global
+                // memory
                 in.lut_store.assign(256, 255);
-                for (int32_t c = 0; c < b->K && c < 255; c++) in.lut_store[size_t(c)] = uint8_t(c);
+                in.lut_t_store.assign(256, 255);
+                for (int64_t c = 0; c < K0 && c < 255; c++) in.lut_store[size_t(c)] = uint8_t(c);
+                for (int64_t c = 0; c < K1 && c < 255; c++) in.lut_t_store[size_t(c)] = uint8_t(c);
                 in.lut = in.lut_store.data();
                 in.lut_len = 256;
+                in.lut_t = in.lut_t_store.data();
+                in.lut_t_len = 256;
                 in.raw_width = 1;
                 in.syms_s = end_s;
                 in.syms_t = end_t;
                 in.bad_s = "out of bounds index in a";
                 in.bad_t = "out of bounds index in b";
+            } else {
+                // host uint16 codes: checked here, per problem, then uploaded
+                const uint16_t *cs = static_cast<const uint16_t *>(b->codes_s);
+                const uint16_t *ct = static_cast<const uint16_t *>(b->codes_t);
+                for (int64_t k = 0; k < b->n; k++) {
+                    for (int64_t q = 0; q < b->len_s[k]; q++)
+                        if (cs[b->off_s[k] + q] >= K0) fail(PA_EINVAL, "out of bounds index in a");
+                    for (int64_t q = 0; q < b->len_t[k]; q++)
+                        if (ct[b->off_t[k] + q] >= K1) fail(PA_EINVAL, "out of bounds index in b");
+                }
+                in.bytes_s = end_s * 2;
+                in.bytes_t = end_t * 2;
+            }
+            if (!b->raw_width && !b->on_device && cb == 1) {
+                in.bytes_s = end_s;
+                in.bytes_t = end_t;
             }
             bool nan_free = true;
-            in.table = make_table(b->table, b->K, b->K, s->opt.dir, &nan_free);
+            in.table = make_table(b->table, K0, K1, s->opt.dir, &nan_free);
             in.nan_free = nan_free;
-            in.K1 = b->K + 1;
+            in.K1 = int(std::max(K0, K1) + 1);
             break;
         }
         case PA_FORM_BINARY:
@@ -1760,6 +1858,7 @@ static void ragged_input(const pa_solver *s, const pa_ragged *b, Input &in) {
     }
     if (b->raw_width) {
         if (b->raw_width != 1 && b->raw_width != 4) fail(PA_EINVAL, "raw symbols are 1 or 4 bytes wide");
+        if (b->form == PA_FORM_INDEXED && b->code_bytes == 2) fail(PA_EINVAL, "raw symbols take no code width");
         if (b->form != PA_FORM_INDEXED && b->form != PA_FORM_BINARY)
             fail(PA_EINVAL, "raw symbols are for the indexed and binary forms");
         in.raw_width = b->raw_width;
@@ -1847,6 +1946,60 @@ int pa_solver_set_gap_costs(pa_solver *s, const float *w_s, int64_t n_s, const f
             HIPCHK(hipMemcpy(s->b_border_t.p, s->border_t.data(), s->border_t.size() * 4, hipMemcpyHostToDevice));
         }
         s->last_plan.valid = false;
+    });
+}
+
+int pa_solver_set_gap_params(pa_solver *s, const double *gap_s, const double *gap_t) {
+    return guard([&] {
+        if (!s || !gap_s || !gap_t) fail(PA_EINVAL, "null argument");
+        pa_options &o = s->opt;
+        if (o.type != PA_TYPE_ALIGNMENT || (o.gap != GAP_LIN && o.gap != GAP_AFF))
+            fail(PA_EINVAL, "the solver's gap costs are not linear or affine");
+        const float sgn = o.dir == PA_MINIMIZE ? 1.0f : -1.0f;
+        // the same float32 conversions as pa_create_options / pa_create_solver
+        if (o.gap == GAP_LIN) {
+            const float ls = float(gap_s[0]), lt = float(gap_t[0]);
+            if (ls < 0 || lt < 0) fail(PA_EINVAL, "gap costs must be non-negative");
+            if (ls == o.lin_s && lt == o.lin_t) return;
+            o.lin_s = ls;
+            o.lin_t = lt;
+            s->go_s = ls * sgn;
+            s->go_t = lt * sgn;
+        } else {
+            const float us = float(gap_s[1]), vs = float(gap_s[0]), ut = float(gap_t[1]), vt = float(gap_t[0]);
+            if (us < 0 || vs < 0 || ut < 0 || vt < 0) fail(PA_EINVAL, "gap costs must be non-negative");
+            if (us == o.aff_u_s && vs == o.aff_v_s && ut == o.aff_u_t && vt == o.aff_v_t) return;
+            o.aff_u_s = us;
+            o.aff_v_s = vs;
+            o.aff_u_t = ut;
+            o.aff_v_t = vt;
+            s->go_s = (us + vs) * sgn;
+            s->ge_s = us * sgn;
+            s->go_t = (ut + vt) * sgn;
+            s->ge_t = ut * sgn;
+        }
+        make_borders(*s);
+        if (!s->has_device) return;
+        // new borders on the solver's stream: after the work already queued
+        // there (which read the old ones), before the next solve's kernels;
+        // staged through pinned memory so the host does not wait
+        const size_t ns = s->border_s.size(), nt = s->border_t.size();
+        if (s->border_pending) HIPCHK(hipEventSynchronize(s->border_ev));
+        s->border_pending = false;
+        if (s->border_stage_cap < ns + nt) {
+            if (s->border_stage) HIPCHK(hipHostFree(s->border_stage));
+            s->border_stage = nullptr;
+            s->border_stage_cap = 0;
+            HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&s->border_stage), (ns + nt) * 4, hipHostMallocDefault));
+            s->border_stage_cap = ns + nt;
+        }
+        if (!s->border_ev) HIPCHK(hipEventCreateWithFlags(&s->border_ev, hipEventDisableTiming));
+        std::memcpy(s->border_stage, s->border_s.data(), ns * 4);
+        std::memcpy(s->border_stage + ns, s->border_t.data(), nt * 4);
+        HIPCHK(hipMemcpyAsync(s->b_border_s.p, s->border_stage, ns * 4, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(s->b_border_t.p, s->border_stage + ns, nt * 4, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipEventRecord(s->border_ev, s->stream));
+        s->border_pending = true;
     });
 }
 
@@ -2196,6 +2349,7 @@ void pa_tape_free(pa_tape *t) {
         give_back(t->seeds, s->b_seeds);
         give_back(t->tb, s->b_tb);
         give_back(t->rowbuf, s->b_rowbuf);
+        give_back(t->borders, s->b_tape_borders);
     }
     delete t;
 }

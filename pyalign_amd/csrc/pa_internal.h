@@ -11,7 +11,8 @@ enum { DIR_MIN = 0, DIR_MAX = 1 };
 enum { GAP_LIN = 0, GAP_AFF = 1, GAP_DTW = 2, GAP_GEN = 3 };
 // Similarity sources.  F_TABLE reads a (K+1)x(K+1) table (row = s symbol,
 // last symbol = pad whose entries are the worst value) staged in LDS with
-// uint8 codes; F_TABLE_G reads it from global memory with uint16 codes.
+// uint8 codes; F_TABLE_G reads it from global memory (L2-resident) with
+// uint16 codes, for tables too large for LDS (more than 128 symbols).
 // F_DROWS is F_DENSE with every problem's rows contiguous (column stride 1)
 // and at least DROWS_MIN_LT columns: each lane stages windows of its rows
 // through LDS with 16-byte loads (a launch-level choice; the payload and
@@ -176,10 +177,11 @@ struct StripWalk {
 hipError_t launch_backward_strips(int gap, int form, int dim, const BwArgs &args, StripWalk *state,
                                   const int32_t *strip_of, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t value, hipStream_t stream);
-// Raw symbols (width 1 or 4 bytes) to codes: uint8 through lut, or uint32
-// values when lut is null; *bad gets the smallest position without a code.
+// Raw symbols (width 1 or 4 bytes) to codes: uint8 or uint16 (out_width 1,
+// 2) through lut, or uint32 values (out_width 4) when lut is null; *bad gets
+// the smallest position without a code.
 hipError_t launch_translate(const void *raw, int width, int64_t count, const uint8_t *lut, int32_t lut_len,
-                            void *codes, unsigned long long *bad, int cus, hipStream_t stream);
+                            void *codes, int out_width, unsigned long long *bad, int cus, hipStream_t stream);
 // General gap costs: one workgroup per problem.  At dvals + dv_off it writes
 // four ls x lt planes: D, D transposed, S and the traceback code of every
 // cell (WSB_* below); then the seed and score, and with `path` set the

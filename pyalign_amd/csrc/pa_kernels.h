@@ -93,6 +93,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                                             int lane) {
     constexpr float W = worst<DIR>();
     constexpr bool DENSE = FORM == F_DENSE || FORM == F_DROWS;
+    constexpr bool TABLE = FORM == F_TABLE || FORM == F_TABLE_G;  // tab: LDS or global memory
     constexpr bool STAGED = FORM == F_DROWS;
     static_assert(!STAGED || R == DROWS_WIN, "staged dense rows need R == 8");
     constexpr int BITS = TR == TR_ALL ? 8 : (TR == TR_ONE ? (GAP == GAP_AFF ? 4 : 2) : 0);
@@ -163,6 +164,10 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
             if constexpr (FORM == F_TABLE) {
                 const uint8_t *cs = (const uint8_t *)a.codes_s;
                 rowsym[r] = (rv[r] ? int(cs[pr.off_s + i]) : PAD) * a.K1;
+            } else if constexpr (FORM == F_TABLE_G) {
+                // a code beyond the table reads the pad row (no read outside it)
+                const uint16_t *cs = (const uint16_t *)a.codes_s;
+                rowsym[r] = (rv[r] ? min(int(cs[pr.off_s + i]), PAD) : PAD) * a.K1;
             } else if constexpr (FORM == F_BINARY) {
                 const uint32_t *cs = (const uint32_t *)a.codes_s;
                 rowsym[r] = rv[r] ? int(cs[pr.off_s + i]) : 0;
@@ -187,6 +192,8 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         // loaded one step ahead, read from these bases at the clamped column.
         [[maybe_unused]] const uint8_t *tb8 =
             act ? (const uint8_t *)a.codes_t + pr.off_t : (const uint8_t *)a.safe;
+        [[maybe_unused]] const uint16_t *tb16 =
+            act ? (const uint16_t *)a.codes_t + pr.off_t : (const uint16_t *)a.safe;
         [[maybe_unused]] const uint32_t *tb32 =
             act ? (const uint32_t *)a.codes_t + pr.off_t : (const uint32_t *)a.safe;
         [[maybe_unused]] const float *tbf = act ? (const float *)a.codes_t + pr.off_t * SDD : a.safe;
@@ -217,6 +224,8 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
             const int jc = act ? (j < 0 ? 0 : (j < lt ? j : lt - 1)) : 0;
             if constexpr (FORM == F_TABLE) {
                 c.sym = int(tb8[jc]);
+            } else if constexpr (FORM == F_TABLE_G) {
+                c.sym = min(int(tb16[jc]), PAD);
             } else if constexpr (FORM == F_BINARY) {
                 c.sym = int(tb32[jc]);
             } else if constexpr (FORM == F_DENSE) {
@@ -355,7 +364,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 for (int r = 0; r < R; ++r) {
                     // similarity S(i, j) (factory.h:573-576, 624-630, 673-679)
                     float S;
-                    if constexpr (FORM == F_TABLE) {
+                    if constexpr (TABLE) {
                         S = tab[rowsym[r] + tsym];
                     } else if constexpr (FORM == F_BINARY) {
                         S = rv[r] ? (rowsym[r] == tsym ? a.eq : a.ne) : W;
@@ -870,6 +879,8 @@ __global__ __launch_bounds__(256) void pa_solve_kernel(KArgs a) {
         for (int k = threadIdx.x; k < n; k += blockDim.x) lds_table[k] = a.table[k];
         __syncthreads();
     }
+    // a table too large for LDS is read where it is (global memory, L2-resident)
+    const float *tab = FORM == F_TABLE_G ? a.table : lds_table;
     const int lane = threadIdx.x & (WAVE - 1);
     // Persistent wavefronts drain the cost-sorted group queue; every wave
     // leaves once its ticket passes the group count.
@@ -881,11 +892,11 @@ __global__ __launch_bounds__(256) void pa_solve_kernel(KArgs a) {
         const Group &g = a.groups[t];
         if constexpr (!FAST && TR != TR_ALL) {
             if (g.strip >= 0) {
-                solve_group<LOC, DIR, GAP, FORM, TR, R, FAST, SD, true>(a, g, lds_table, lane);
+                solve_group<LOC, DIR, GAP, FORM, TR, R, FAST, SD, true>(a, g, tab, lane);
                 continue;
             }
         }
-        solve_group<LOC, DIR, GAP, FORM, TR, R, FAST, SD>(a, g, lds_table, lane);
+        solve_group<LOC, DIR, GAP, FORM, TR, R, FAST, SD>(a, g, tab, lane);
     }
 }
 
@@ -985,6 +996,10 @@ static hipError_t dispatch_lg(const LaunchSpec &s, const KArgs &a, hipStream_t s
             case F_TABLE:
                 if (s.tr == TR_ONE) PA_GO(F_TABLE, TR_ONE, 8, false, 0);
                 PA_GO(F_TABLE, TR_ALL, 8, false, 0);
+            case F_TABLE_G:
+                if (s.tr == TR_SCORE) PA_GO(F_TABLE_G, TR_SCORE, 8, false, 0);
+                if (s.tr == TR_ONE) PA_GO(F_TABLE_G, TR_ONE, 8, false, 0);
+                PA_GO(F_TABLE_G, TR_ALL, 8, false, 0);
             case F_BINARY:
                 if (s.tr == TR_SCORE) PA_GO(F_BINARY, TR_SCORE, 8, false, 0);
                 if (s.tr == TR_ONE) PA_GO(F_BINARY, TR_ONE, 8, false, 0);

@@ -35,6 +35,7 @@ int occupancy_waves(const LaunchSpec &s, int dim) {
     return waves;
 }
 
+// F_TABLE_G (tables read from global memory) is built at R = 8 only
 int band_R(int form, int tr) {
     const bool r4 = (form == F_TABLE && (tr == TR_SCORE || tr == TR_ONE)) ||
                     (form == F_BINARY && (tr == TR_SCORE || tr == TR_ONE));
@@ -899,6 +900,11 @@ __global__ __launch_bounds__(WSB_MAX_THREADS) void pa_wsb_kernel(KArgs a, const 
                 const int x = ((const uint8_t *)a.codes_s)[pr.off_s + i];
                 const int y = ((const uint8_t *)a.codes_t)[pr.off_t + j];
                 S = a.table[x * a.K1 + y];
+            } else if constexpr (FORM == F_TABLE_G) {
+                // codes beyond the table read its pad row / column
+                const int x = min(int(((const uint16_t *)a.codes_s)[pr.off_s + i]), a.K1 - 1);
+                const int y = min(int(((const uint16_t *)a.codes_t)[pr.off_t + j]), a.K1 - 1);
+                S = a.table[int64_t(x) * a.K1 + y];
             } else {
                 const uint32_t x = ((const uint32_t *)a.codes_s)[pr.off_s + i];
                 const uint32_t y = ((const uint32_t *)a.codes_t)[pr.off_t + j];
@@ -1036,6 +1042,7 @@ hipError_t launch_wsb(int loc, int dir, int form, const KArgs &a, int64_t n, con
     do {                                                        \
         if (form == F_DENSE) PA_WSB(LC, DR, F_DENSE);           \
         else if (form == F_TABLE) PA_WSB(LC, DR, F_TABLE);      \
+        else if (form == F_TABLE_G) PA_WSB(LC, DR, F_TABLE_G);  \
         else if (form == F_BINARY) PA_WSB(LC, DR, F_BINARY);    \
         else return hipErrorInvalidValue;                       \
     } while (0)
@@ -1053,14 +1060,15 @@ hipError_t launch_wsb(int loc, int dir, int form, const KArgs &a, int64_t n, con
 // ---------------------------------------------------------------------------
 // Problem encoding on the device: raw symbols of a whole bag (1-byte latin-1
 // or 4-byte code points, the text the host joined) become the solver's codes,
-// through `lut` into uint8 table codes for the indexed form, or widened to
-// uint32 values for the binary form (lut == nullptr).  A symbol without a
-// code (beyond lut_len, or 255 in lut) writes code 0 and leaves the smallest
-// such position in *bad, which the host checks before any DP launch.
+// through `lut` into table codes for the indexed form (uint8, or uint16 for
+// tables read from global memory), or widened to uint32 values for the
+// binary form (lut == nullptr).  A symbol without a code (beyond lut_len, or
+// 255 in lut) writes code 0 and leaves the smallest such position in *bad,
+// which the host checks before any DP launch.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void pa_translate_kernel(const void *raw, int width, int64_t count,
                                                            const uint8_t *lut, int32_t lut_len, void *codes,
-                                                           unsigned long long *bad) {
+                                                           int out_width, unsigned long long *bad) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < count; q += stride) {
         const uint32_t x = width == 1 ? uint32_t(static_cast<const uint8_t *>(raw)[q])
@@ -1071,7 +1079,8 @@ __global__ __launch_bounds__(256) void pa_translate_kernel(const void *raw, int 
                 atomicMin(bad, static_cast<unsigned long long>(q));
                 c = 0;
             }
-            static_cast<uint8_t *>(codes)[q] = uint8_t(c);
+            if (out_width == 2) static_cast<uint16_t *>(codes)[q] = uint16_t(c);
+            else static_cast<uint8_t *>(codes)[q] = uint8_t(c);
         } else {
             static_cast<uint32_t *>(codes)[q] = x;
         }
@@ -1079,12 +1088,13 @@ __global__ __launch_bounds__(256) void pa_translate_kernel(const void *raw, int 
 }
 
 hipError_t launch_translate(const void *raw, int width, int64_t count, const uint8_t *lut, int32_t lut_len,
-                            void *codes, unsigned long long *bad, int cus, hipStream_t st) {
+                            void *codes, int out_width, unsigned long long *bad, int cus, hipStream_t st) {
     if (count == 0) return hipSuccess;
     if (width != 1 && width != 4) return hipErrorInvalidValue;
+    if (lut ? (out_width != 1 && out_width != 2) : out_width != 4) return hipErrorInvalidValue;
     const int64_t blocks = std::min<int64_t>((count + 255) / 256, int64_t(std::max(cus, 1)) * 16);
     hipLaunchKernelGGL(pa_translate_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, raw, width,
-                       count, lut, lut_len, codes, bad);
+                       count, lut, lut_len, codes, out_width, bad);
     return hipGetLastError();
 }
 

@@ -580,10 +580,7 @@ class Solver:
             if text is not None:
                 # str sequences: the device encodes the joined text
                 if bag.form == Form.INDEXED_MATRIX_FORM:
-                    table = _shared_table(bag)
-                    if table.shape[0] > 128 or table.shape[1] > 128:
-                        raise ValueError("alphabets above 128 symbols are not supported by the MI355X indexed form")
-                    payload["table"] = np.asarray(table, dtype=np.float32)
+                    payload["table"] = np.asarray(_shared_table(bag), dtype=np.float32)
                     form = "indexed"
                 else:
                     payload["eq"], payload["ne"] = _shared_eq_ne(bag)
@@ -592,13 +589,12 @@ class Solver:
                 table = _shared_table(bag)
                 cs, os_, _ = _encode_bag(problems, 0)
                 ct, ot, _ = _encode_bag(problems, 1)
-                if table.shape[0] > 128 or table.shape[1] > 128:
-                    raise ValueError("alphabets above 128 symbols are not supported by the MI355X indexed form")
                 if cs.size and cs.max() >= table.shape[0]:
                     raise ValueError("out of bounds index in a")
                 if ct.size and ct.max() >= table.shape[1]:
                     raise ValueError("out of bounds index in b")
-                payload = dict(codes_s=cs.astype(np.uint8), codes_t=ct.astype(np.uint8), off_s=os_, off_t=ot,
+                code_t = np.uint16 if max(table.shape) > 128 else np.uint8
+                payload = dict(codes_s=cs.astype(code_t), codes_t=ct.astype(code_t), off_s=os_, off_t=ot,
                                table=np.asarray(table, dtype=np.float32))
                 form = "indexed"
             elif bag.form == Form.BINARY_MATRIX_FORM:

@@ -57,14 +57,25 @@ _SOLVERS = {}  # recent solvers by configuration (a tape keeps its own buffers)
 _MAX_SOLVERS = 8
 
 
-def _solver(kind, direction, locality, gap, gap_open, gap_extend, ls, lt):
+def _gap_cost(gap, gap_open, gap_extend):
+    return AffineGapCost(float(gap_open), float(gap_extend)) if gap == "affine" else LinearGapCost(float(gap_open))
+
+
+def _solver(kind, direction, locality, gap, gap_open, gap_extend, ls, lt, device=None):
+    """A cached solver for this configuration.  The gap values are not part
+    of the key: learnable gap costs change every optimisation step, and the
+    solver takes the new values in place (same buffers, plan and scratch;
+    only its border vectors are recomputed, on torch's current stream)."""
     rounder = SizeRounder()
     lim_s = rounder(int(ls.max()) if len(ls) else 0)
     lim_t = rounder(int(lt.max()) if len(lt) else 0)
-    key = (kind, direction, locality, gap, gap_open, gap_extend, lim_s, lim_t)
+    key = (kind, direction, locality, gap, lim_s, lim_t)
     solver = _SOLVERS.pop(key, None)
     if solver is None:
         solver = _new_solver(kind, direction, locality, gap, gap_open, gap_extend, lim_s, lim_t)
+    elif kind != "dtw":
+        solver.set_stream(torch.cuda.current_stream(device).cuda_stream)
+        solver.set_gap_cost(_gap_cost(gap, gap_open, gap_extend))
     _SOLVERS[key] = solver  # most recent last
     while len(_SOLVERS) > _MAX_SOLVERS:
         _SOLVERS.pop(next(iter(_SOLVERS)))
@@ -80,8 +91,7 @@ def _new_solver(kind, direction, locality, gap, gap_open, gap_extend, lim_s, lim
     else:
         opts["solver"] = algorithm.Type.ALIGNMENT
         opts["locality"] = _LOC[locality]
-        opts["gap_cost"] = (AffineGapCost(float(gap_open), float(gap_extend)) if gap == "affine"
-                            else LinearGapCost(float(gap_open)))
+        opts["gap_cost"] = _gap_cost(gap, gap_open, gap_extend)
     return bits.create_solver(lim_s, lim_t, bits.create_options(opts))
 
 
@@ -109,7 +119,7 @@ class _AlignScore(torch.autograd.Function):
         lt = _lengths(len_t, n)
         if (ls > Ls).any() or (lt > Lt).any():
             raise ValueError("lengths exceed the padded shape of S")
-        solver = _solver("align", direction, locality, gap, float(gap_open), float(gap_extend), ls, lt)
+        solver = _solver("align", direction, locality, gap, float(gap_open), float(gap_extend), ls, lt, S.device)
         solver.set_checkpoint(checkpoint)
         dense_off = np.arange(n, dtype=np.int64) * (Ls * Lt)
         rg, keep = algorithm.make_ragged("dense", ls, lt, dense=S.data_ptr(), dense_off=dense_off,

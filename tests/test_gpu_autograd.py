@@ -296,3 +296,61 @@ def test_tapes_share_the_solvers_buffers():
     with pytest.raises(ValueError, match="solver was freed"):
         check(lib().pa_backward(h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(dS.data_ptr()), None, None, None))
     lib().pa_tape_free(h)
+
+
+def test_learnable_gap_costs_reuse_one_solver():
+    """A training loop whose gap costs change every step keeps one cached
+    solver (its border vectors recomputed in place), holds device memory
+    flat and gives the scores and gradients of a fresh solver bit for bit,
+    including a checkpointed pair (recomputed in the backward from the
+    tape's own borders) and two tapes of different gap costs alive at once."""
+    from pyalign_amd import torch_ops
+    rng = np.random.default_rng(65)
+    S, ls, lt = c4_batch(rng, 12, 40, 300, dim=16)
+    # one long global pair, recomputed strip by strip in the backward
+    S = np.pad(S, ((0, 1), (0, 2100 - S.shape[1]), (0, 0)))
+    S[-1, :2100, :] = (rng.standard_normal((2100, S.shape[2])) / 8).astype(np.float32)
+    ls = np.append(ls, 2100).astype(np.int32)
+    lt = np.append(lt, S.shape[2]).astype(np.int32)
+    St = torch.tensor(S, device="cuda")
+
+    def step(o, e, backward=True):
+        Sg = St.clone().requires_grad_(True)
+        go = torch.tensor(o, requires_grad=True)
+        ge = torch.tensor(e, requires_grad=True)
+        out = align_score(Sg, ls, lt, go, ge, locality="global")
+        if not backward:
+            return out, Sg, go, ge
+        out.sum().backward()
+        return (out.detach().cpu().numpy(), Sg.grad.cpu().numpy(), go.grad.item(), ge.grad.item())
+
+    def same(x, y):
+        return (np.array_equal(x[0].view(np.uint32), y[0].view(np.uint32))
+                and np.array_equal(x[1].view(np.uint32), y[1].view(np.uint32)) and x[2] == y[2] and x[3] == y[3])
+
+    gaps = [(1.0 + 0.013 * k, 0.2 + 0.0031 * k) for k in range(50)]
+    torch_ops._SOLVERS.clear()
+    kept, free = {}, []
+    for k, (o, e) in enumerate(gaps):
+        r = step(o, e)
+        if k in (0, 17, 49):
+            kept[k] = r
+        assert len(torch_ops._SOLVERS) == 1
+        torch.cuda.synchronize()
+        free.append(torch.cuda.mem_get_info()[0])
+    assert free[10] == free[-1], (free[10], free[-1])
+    for k, r in kept.items():
+        torch_ops._SOLVERS.clear()
+        assert same(r, step(*gaps[k])), k
+    # two live tapes of different gap costs, backwards in reverse order
+    torch_ops._SOLVERS.clear()
+    fresh_a = step(*gaps[3])
+    torch_ops._SOLVERS.clear()
+    fresh_b = step(*gaps[40])
+    torch_ops._SOLVERS.clear()
+    a = step(*gaps[3], backward=False)
+    b = step(*gaps[40], backward=False)
+    for (out, Sg, go, ge), fresh in ((b, fresh_b), (a, fresh_a)):
+        out.sum().backward()
+        got = (out.detach().cpu().numpy(), Sg.grad.cpu().numpy(), go.grad.item(), ge.grad.item())
+        assert same(got, fresh)

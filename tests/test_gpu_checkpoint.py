@@ -126,8 +126,9 @@ def test_minimizing_linear_checkpointed():
 
 def test_dtw_full_size_batch():
     """8,192 DTW pairs, lengths U[128,4096] (3-D random walks): distances and
-    gradients with checkpointed strips equal the full-traceback run, and two
-    pairs (one checkpointed, one not) match the oracle's optimal path."""
+    gradients with checkpointed strips equal the full-traceback run, and 66
+    pairs (the longest, the shortest and 64 spread over the batch, most of
+    them checkpointed) match the oracle's distance and optimal path."""
     import bench
     import oracle as O
     import pyalign_amd as pa
@@ -142,7 +143,9 @@ def test_dtw_full_size_batch():
     lims = (pa.solve.SizeRounder()(int(lx.max())), pa.solve.SizeRounder()(int(ly.max())))
     longest = int(np.argmax(np.maximum(lx, ly)))
     short = int(np.argmin(np.maximum(lx, ly)))
-    for k in (longest, short):
+    # the longest and shortest pairs and 64 more spread over the batch
+    sample = [longest, short] + list(range(61, len(lx), len(lx) // 64))[:64]
+    for k in sample:
         p = pf.new_problem(X[k, :lx[k]], Y[k, :ly[k]])
         ref = O.OracleResult(O.make_config(solver_type=O.DTW, direction=O.MINIMIZE, goal=O.GOAL_ONE,
                                            lim_s=lims[0], lim_t=lims[1]), p.matrix)

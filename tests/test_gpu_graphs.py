@@ -4,7 +4,8 @@ A solver captures a launch sequence the second time it sees it (same
 shapes, same buffers) and replays the graph afterwards.  Every replay is
 checked against an eager solve of the same inputs (graphs disabled), with
 other eager GPU work and synchronisations between the replays, and the
-inputs changed in place between replays (the graph must read the new data).
+inputs changed in place between replays (the graph must read the new data);
+a sample of every checked replay's scores and paths against the oracle.
 """
 
 import ctypes
@@ -13,6 +14,7 @@ import numpy as np
 import pytest
 import torch
 
+import oracle as O
 import pyalign_amd as pa
 from pyalign_amd import _lib, algorithm
 from pyalign_amd._lib import check, lib
@@ -65,12 +67,19 @@ def test_replayed_device_scores_match_eager(lengths):
     other = batch(rng, 50, 20, 90)
     for rep in range(6):
         if rep == 3:  # new data in the same buffers: replays must read it
-            d_cs.copy_(torch.from_numpy(rng.integers(0, 20, cs.size).astype(np.uint8)).cuda())
+            cs = rng.integers(0, 20, cs.size).astype(np.uint8)
+            d_cs.copy_(torch.from_numpy(cs).cuda())
             torch.cuda.synchronize()
         got = device_scores(graph_solver, ls, lt, d_cs, d_ct, os_, ot, table, d_out)
         want = device_scores(eager_solver, ls, lt, d_cs, d_ct, os_, ot, table,
                              torch.empty(n, dtype=torch.float32, device="cuda"))
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), rep
+        if rep in (2, 3, 5):  # replays (and the capture after new data) against the oracle
+            for k in range(0, n, max(1, n // 25)):
+                S = table[np.ix_(cs[os_[k]:os_[k] + ls[k]], ct[ot[k]:ot[k] + lt[k]])]
+                ref = O.OracleResult(O.make_config(locality=O.LOCAL, gap=("affine", 10, 1), goal=O.GOAL_SCORE,
+                                                   lim_s=lim, lim_t=lim), S)
+                assert got[k].tobytes() == ref.score.tobytes(), (rep, k)
         if lengths[1] >= 1000:
             assert graph_solver.last_schedule()[1] == n
         # unrelated eager work between replays
@@ -93,5 +102,11 @@ def test_replayed_solutions_match_eager():
         for g, w in zip(got, want):
             assert np.float32(g.score).tobytes() == np.float32(w.score).tobytes(), rep
             assert np.array_equal(np.asarray(g.path), np.asarray(w.path)), rep
+        for k in range(rep, len(ls), 23):  # replayed paths against the oracle
+            S = table[np.ix_(cs[os_[k]:os_[k] + ls[k]], ct[ot[k]:ot[k] + lt[k]])]
+            ref = O.OracleResult(O.make_config(locality=O.LOCAL, gap=("affine", 10, 1), goal=O.GOAL_ONE,
+                                               lim_s=256, lim_t=256), S)
+            assert np.float32(got[k].score).tobytes() == ref.score.tobytes(), (rep, k)
+            assert np.array_equal(np.asarray(got[k].path, dtype=np.int32), ref.path()), (rep, k)
         x = torch.randn(1000, device="cuda").sum().item()  # eager torch work between replays
         assert np.isfinite(x)

@@ -364,3 +364,56 @@ def test_dtw_series_long_multi_strip():
         ref = _dtw_oracle(p.matrix, O.MINIMIZE, O.GOAL_ONE, lims)
         assert np.float32(r.score).tobytes() == ref.score.tobytes()
         assert np.array_equal(np.asarray(r.path, dtype=np.int32), ref.path())
+
+
+def test_c2_full_length_range_against_oracle():
+    """The batched local Gotoh workload's generator over its whole length
+    range, [50, 2000]: 4,096 pairs solved in one bag on the fast kernel
+    (every strip height it serves), every 8th pair against the oracle."""
+    import bench
+    table, ls, lt, cs, ct, off_s, off_t = bench.c2_inputs(n=4096, seed=12)
+    assert ls.max() > 1800 and lt.max() > 1800
+    opts = algorithm.m16.create_options(dict(
+        solver=algorithm.Type.ALIGNMENT, batch=True, direction=algorithm.Direction.MAXIMIZE, return_dup=False,
+        codomain=pa.solve.Codomain(pa.Score), locality=algorithm.Locality.LOCAL, gap_cost=pa.AffineGapCost(10, 1)))
+    lim = int(max(ls.max(), lt.max()))
+    solver = algorithm.m16.create_solver(lim, lim, opts)
+    got = solver.solve_ragged("score", "indexed", ls, lt, codes_s=cs, codes_t=ct, off_s=off_s, off_t=off_t,
+                              table=table)
+    for k in range(0, len(ls), 8):
+        S = table[np.ix_(cs[off_s[k]:off_s[k] + ls[k]], ct[off_t[k]:off_t[k] + lt[k]])]
+        ref = O.OracleResult(O.make_config(locality=O.LOCAL, gap=("affine", 10, 1), goal=O.GOAL_SCORE,
+                                           lim_s=lim, lim_t=lim), S)
+        assert np.float32(got[k]).tobytes() == ref.score.tobytes(), k
+
+
+@pytest.mark.parametrize("loc", list(SOLVERS))
+def test_return_dup_all_optimal_in_order(loc):
+    """return_dup=True: the DFS emits every path, duplicates of an already
+    emitted compressed alignment included (options.h:21,
+    solver.h:1705-1711), in the oracle's order."""
+    rng = np.random.default_rng(zlib.crc32(("dup" + loc).encode()))
+    alphabet, pf = build_factory("alphabetic", rng, "maximize")
+    seqs, probs = random_bag(rng, pf, alphabet, 25, 0, 8)
+    cls, oloc = SOLVERS[loc]
+    lims = (rounded(max(len(a) for a, _ in seqs)), rounded(max(len(b) for _, b in seqs)))
+    dups = 0
+    for gap in (("linear", 1), ("affine", 2, 1)):
+        its = cls(gap_cost=gap_obj(gap), codomain=typing.Iterator[pa.Solution], return_dup=True).solve(probs)
+        for p, it in zip(probs, its):
+            ref = O.OracleResult(O.make_config(locality=oloc, direction=O.MAXIMIZE, gap=gap, goal=O.GOAL_ALL,
+                                               remove_dup=False, lim_s=lims[0], lim_t=lims[1]), dense(p))
+            expect = []
+            while True:
+                x = ref.all_next()
+                if x is None:
+                    break
+                expect.append(x)
+            got = list(it)
+            assert len(got) == len(expect), (p.s, p.t)
+            keys = [tuple(map(tuple, e.tolist())) for _, _, e in expect]
+            dups += len(keys) - len(set(keys))
+            for g, (score, path, edges) in zip(got, expect):
+                assert np.float32(g.score).tobytes() == np.float32(score).tobytes()
+                assert np.array_equal(np.asarray(g.path, dtype=np.int32), path)
+    assert dups > 0  # duplicates were emitted, not deduplicated
