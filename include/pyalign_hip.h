@@ -201,6 +201,24 @@ int pa_solve_ragged_scores_device(pa_solver *solver, const pa_ragged *batch, int
                                   float *scores_device);
 int pa_solver_sync(pa_solver *solver);
 
+/* Allocation-free score solve (no reference counterpart: it is for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same results as
+ * pa_solve_ragged_scores_device, but every buffer its kernels use besides the
+ * caller's inputs and outputs (problem and work-group descriptors, queue
+ * tickets, borders, the similarity table, the strip-boundary row buffer)
+ * lives in one static device array of 64 MiB per GPU, held until the process
+ * ends; the call allocates nothing (no hipMalloc, hipFree, pinned memory,
+ * stream or event of its own) and fails with PA_EINVAL when a batch needs
+ * more scratch than that.  The payload must be device-resident (on_device =
+ * 1, codes given, no raw symbols); general gap costs are not served.  Work
+ * goes to `stream` (NULL: the null stream) without a host wait.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams or threads) race, and a call's scratch holds its data only until
+ * the next call. */
+int pa_solve_ragged_scores_noalloc(pa_solver *solver, const pa_ragged *batch, int32_t schedule,
+                                   float *scores_device, void *stream);
+
 /* Timing of the last solve's DP kernel launches (HIP events on the solver's
  * stream): total milliseconds and launch count. */
 int pa_solver_last_kernel_ms(const pa_solver *solver, double *ms, int32_t *launches,

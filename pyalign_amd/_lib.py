@@ -62,6 +62,7 @@ SIGNATURES = {
     "pa_solve_ragged": (ctypes.c_int, [_P, ctypes.POINTER(Ragged), _I32, ctypes.POINTER(_P)]),
     "pa_solve_ragged_scores_device": (ctypes.c_int, [_P, ctypes.POINTER(Ragged), _I32, _P]),
     "pa_solver_sync": (ctypes.c_int, [_P]),
+    "pa_solve_ragged_scores_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(Ragged), _I32, _P, _P]),
     "pa_solver_last_kernel_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double),
                                                 ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_double)]),
     "pa_solver_set_graphs": (ctypes.c_int, [_P, _I32]),

@@ -1900,6 +1900,137 @@ int pa_solve_ragged_scores_device(pa_solver *s, const pa_ragged *batch, int32_t 
     });
 }
 
+int pa_solve_ragged_scores_noalloc(pa_solver *s, const pa_ragged *batch, int32_t schedule, float *scores_device,
+                                   void *stream) {
+    return guard([&] {
+        if (!s || !batch || !scores_device) fail(PA_EINVAL, "null argument");
+        const pa_options &o = s->opt;
+        if (o.goal != PA_GOAL_SCORE && o.type != PA_TYPE_DTW)
+            fail(PA_EINVAL, "device score output needs a score-only solver");
+        if (o.gap == GAP_GEN) fail(PA_EINVAL, "general gap costs need the allocating entry points");
+        if (!batch->on_device || batch->raw_width)
+            fail(PA_EINVAL, "the allocation-free solve takes device-resident codes, series or similarities");
+        Input in{};
+        ragged_input(s, batch, in);
+        in.schedule = schedule;
+        const int64_t n = in.n;
+        for (int64_t k = 0; k < n; k++) {
+            if (in.len_s[k] < 0 || in.len_t[k] < 0) fail(PA_EINVAL, "negative sequence length");
+            if (in.len_s[k] > s->lim_s || in.len_t[k] > s->lim_t) {
+                std::ostringstream e;
+                e << "sequence " << (in.len_s[k] > s->lim_s ? "s" : "t") << " of length "
+                  << (in.len_s[k] > s->lim_s ? in.len_s[k] : in.len_t[k]) << " exceeds configured maximum length "
+                  << (in.len_s[k] > s->lim_s ? s->lim_s : s->lim_t);
+                fail(PA_ELENGTH, e.str());
+            }
+        }
+        Job job;
+        job.loc = o.type == PA_TYPE_DTW ? LOC_GLOBAL : o.loc;
+        job.dir = o.dir == PA_MAXIMIZE ? DIR_MAX : DIR_MIN;
+        job.gap = o.gap;
+        job.tr = TR_SCORE;
+        switch (in.form) {
+            case PA_FORM_DENSE: job.form = F_DENSE; break;
+            case PA_FORM_INDEXED: job.form = in.table_g ? F_TABLE_G : F_TABLE; break;
+            case PA_FORM_BINARY: job.form = F_BINARY; break;
+            case PA_FORM_SERIES: job.form = F_SERIES; break;
+            default: fail(PA_EINVAL, "unknown problem form");
+        }
+        if (job.form == F_SERIES && o.type != PA_TYPE_DTW) fail(PA_EINVAL, "the series form is for DTW solvers");
+        if (o.type == PA_TYPE_DTW && job.form != F_DENSE && job.form != F_SERIES)
+            fail(PA_EINVAL, "DTW solvers take dense costs or series");
+        if (job.form == F_SERIES && (in.dim < 1 || in.dim > 4)) fail(PA_EINVAL, "series dimension must be 1..4");
+        job.fast = job.form == F_TABLE && job.loc == LOC_LOCAL && job.dir == DIR_MAX && o.gap != GAP_DTW &&
+                   in.nan_free;
+
+        void *base = nullptr;
+        int cus = 0;
+        HIPCHK(noalloc_scratch(&base, &cus));
+        if (!s->has_device) s->cus = cus;
+        // no band mode: its strips hand rows down through progress counters
+        // whose failure flag the host would have to read back
+        Plan plan = in.schedule == PA_SCHED_PLAIN ? plan_plain(job, in.len_s, in.len_t, n)
+                                                  : plan_auto(job, in.len_s, in.len_t, n, cus * 4, nullptr, false);
+        place_plan(plan, in, 0, false, 1);
+
+        // the scratch layout: safe values, sink, queue tickets, borders,
+        // table, problem and group descriptors, strip-boundary row buffer
+        size_t at = 0;
+        auto take = [&](size_t bytes) {
+            const size_t here = at;
+            at = (at + bytes + 255) / 256 * 256;
+            return here;
+        };
+        const size_t o_safe = take(SAFE_FLOATS * 4);
+        const size_t o_sink = take(SINK_WORDS * 4);
+        const size_t o_queue = take(std::max<size_t>(plan.launches.size(), 1) * 4);
+        const size_t o_bs = take(s->border_s.size() * 4);
+        const size_t o_bt = take(s->border_t.size() * 4);
+        const size_t o_table = take(in.table.size() * 4);
+        const size_t o_probs = take(plan.probs.size() * sizeof(Prob));
+        const size_t o_groups = take(plan.groups.size() * sizeof(Group));
+        const size_t o_rowbuf = take(size_t(plan.rowbuf) * sizeof(float2));
+        if (at > NOALLOC_SCRATCH_BYTES) {
+            std::ostringstream e;
+            e << "the allocation-free solve needs " << at << " bytes of device scratch for this batch; it holds "
+              << NOALLOC_SCRATCH_BYTES << " (use pa_solve_ragged_scores_device, or smaller batches)";
+            fail(PA_EINVAL, e.str());
+        }
+        char *dev = static_cast<char *>(base);
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        float safe[SAFE_FLOATS] = {};
+        safe[0] = -std::numeric_limits<float>::infinity();
+        safe[1] = std::numeric_limits<float>::infinity();
+        // pageable sources: each copy has read its source when the call returns
+        auto put = [&](size_t off, const void *src, size_t bytes) {
+            if (bytes) HIPCHK(hipMemcpyAsync(dev + off, src, bytes, hipMemcpyHostToDevice, st));
+        };
+        put(o_safe, safe, sizeof(safe));
+        put(o_bs, s->border_s.data(), s->border_s.size() * 4);
+        put(o_bt, s->border_t.data(), s->border_t.size() * 4);
+        put(o_table, in.table.data(), in.table.size() * 4);
+        put(o_probs, plan.probs.data(), plan.probs.size() * sizeof(Prob));
+        put(o_groups, plan.groups.data(), plan.groups.size() * sizeof(Group));
+
+        KArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.probs = reinterpret_cast<const Prob *>(dev + o_probs);
+        a.groups = reinterpret_cast<const Group *>(dev + o_groups);
+        a.codes_s = in.codes_s;
+        a.codes_t = in.codes_t;
+        a.table = reinterpret_cast<const float *>(dev + o_table);
+        a.K1 = in.K1;
+        a.dim = in.dim;
+        a.eq = in.eq;
+        a.ne = in.ne;
+        a.dense = in.dense;
+        a.rowbuf = reinterpret_cast<float2 *>(dev + o_rowbuf);
+        a.border_s = reinterpret_cast<const float *>(dev + o_bs);
+        a.border_t = reinterpret_cast<const float *>(dev + o_bt);
+        a.safe = reinterpret_cast<const float *>(dev + o_safe);
+        a.sink = reinterpret_cast<uint32_t *>(dev + o_sink);
+        a.go_s = s->go_s;
+        a.ge_s = s->ge_s;
+        a.go_t = s->go_t;
+        a.ge_t = s->ge_t;
+        a.scores = scores_device;
+        a.val_layers = 1;
+        uint32_t *queue = reinterpret_cast<uint32_t *>(dev + o_queue);
+        std::vector<LaunchSpec> specs;
+        std::vector<KArgs> largs;
+        build_launches(*s, job, in, plan, a, queue, specs, largs);
+        // the queue tickets start at zero (a fill kernel: the last call's
+        // counts are still there)
+        hipError_t e = launch_zero_u32(queue, int64_t(std::max<size_t>(plan.launches.size(), 1)), st);
+        for (size_t q = 0; q < specs.size() && e == hipSuccess; q++) e = launch_solve(specs[q], largs[q], st);
+        if (e != hipSuccess) {
+            std::ostringstream m;
+            m << "kernel launch failed (allocation-free solve): " << hipGetErrorString(e);
+            fail(PA_EHIP, m.str());
+        }
+    });
+}
+
 int pa_solver_sync(pa_solver *s) {
     return guard([&] {
         if (!s) fail(PA_EINVAL, "null solver");

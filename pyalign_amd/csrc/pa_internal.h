@@ -196,6 +196,11 @@ hipError_t launch_wsb(int loc, int dir, int form, const KArgs &args, int64_t n, 
                       int32_t *path, const int64_t *path_off, int32_t *path_len, int threads,
                       hipStream_t stream);
 bool supported_R(int form, int tr, bool fastmax, int R);
+// The allocation-free entry point's device memory: a file-scope device array
+// of NOALLOC_SCRATCH_BYTES per GPU.  noalloc_scratch gives the current
+// device's copy (looked up once per device) and its CU count.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+hipError_t noalloc_scratch(void **ptr, int *cus);
 // strip height of band-mode strips (long problems): 4 where the general
 // kernel is built for it (more, shorter strips on concurrent wavefronts), else 8
 int band_R(int form, int tr);

@@ -3,6 +3,7 @@
 #include "pa_kernels.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace pa {
 
@@ -1110,5 +1111,43 @@ hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t value, hipStream_t s
 }
 
 hipError_t launch_zero_u32(uint32_t *p, int64_t n, hipStream_t st) { return launch_fill_u32(p, n, 0u, st); }
+
+// ---------------------------------------------------------------------------
+// Device memory of the allocation-free entry point: every GPU the library
+// runs on holds one copy of this array for the life of the process.  Host
+// code never passes the array's name to a kernel or a copy (on the host it is
+// only a stand-in for the symbol): the address of the current device's copy
+// comes from hipGetSymbolAddress, looked up once per device and kept.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+int g_scratch_cus[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(void **ptr, int *cus) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        int n = 0;
+        e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        g_scratch_cus[dev] = n;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    *cus = g_scratch_cus[dev];
+    return hipSuccess;
+}
 
 }  // namespace pa
