@@ -393,8 +393,13 @@ def test_return_dup_all_optimal_in_order(loc):
     emitted compressed alignment included (options.h:21,
     solver.h:1705-1711), in the oracle's order."""
     rng = np.random.default_rng(zlib.crc32(("dup" + loc).encode()))
-    alphabet, pf = build_factory("alphabetic", rng, "maximize")
+    alphabet = "ACGT"
+    pf = pa.alphabetic(alphabet, lambda x, y: 3.0 if x == y else -5.0)
     seqs, probs = random_bag(rng, pf, alphabet, 25, 0, 8)
+    # A-A, then G / T bridged by one gap on each side in either order (one
+    # compressed alignment reached by two paths), then C-C
+    seqs.append(("AGC", "ATC"))
+    probs.append(pf.new_problem("AGC", "ATC"))
     cls, oloc = SOLVERS[loc]
     lims = (rounded(max(len(a) for a, _ in seqs)), rounded(max(len(b) for _, b in seqs)))
     dups = 0
