@@ -141,7 +141,10 @@ struct Plan {
     int64_t dvals = 0;
     double cells = 0.0;
     int64_t band_slots = 0;            // strips of band-mode problems
-    std::vector<int32_t> band_probs;   // their indices in probs
+    std::vector<int32_t> band_probs;   // their indices in probs (long-mode problems first)
+    int32_t n_long = 0;                // long-mode problems (pa_long_kernel): band_probs[0 .. n_long)
+    int32_t long_bands = 0;            // their bands (workgroup tickets)
+    int64_t xbuf_begin = 0, xbuf_end = 0;  // float2 range of the row buffer holding their tagged hand-offs
 };
 
 // The last call's schedule, reused (with its device copy) while the next
@@ -478,8 +481,19 @@ struct Job {
 // ckpt[k] != 0: problem k keeps no traceback (a checkpointed forward): one
 // wavefront sweeps its strips of BAND_L x BAND_R rows in order and keeps
 // every strip's bottom row, from which the backward recomputes it.
+// Long mode (pa_long_kernel) for a problem that would otherwise be the
+// critical path: its strips of 64 rows on the waves of workgroups, LONG_DW
+// phases of LONG_T steps apart within a workgroup (rows handed down through
+// LDS) and about twice that between workgroups, so the serial time is
+// (len_t + 63 + (strips - 1) * LONG_DW * LONG_T * ~1.25) steps of one row.
+double long_serial(int64_t ls, int64_t lt, int *nst_out) {
+    const int64_t nst = (ls + WAVE * LONG_R - 1) / (WAVE * LONG_R);
+    *nst_out = int(nst);
+    return double(lt + WAVE - 1 + double(nst - 1) * LONG_DW * LONG_T * 1.25) * (STEP_OVERHEAD + LONG_R);
+}
+
 Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, int simds,
-               const uint8_t *ckpt = nullptr, bool allow_band = true) {
+               const uint8_t *ckpt = nullptr, bool allow_band = true, bool allow_long = true) {
     std::vector<int> Rs;
     if (job.fast) {
         Rs.assign(std::begin(FAST_RS), std::end(FAST_RS));
@@ -522,8 +536,10 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         bool ckpt;    // a checkpointed problem (strip rows kept, no traceback)
     };
     const bool band_ok = allow_band && job.tr != TR_ALL && supported_R(job.form, job.tr, false, BAND_R);
+    const bool long_ok = allow_band && allow_long && !std::getenv("PYALIGN_AMD_NO_LONG_MODE") &&
+                         long_supported(job.gap, job.form, job.tr);
     std::vector<Item> items(static_cast<size_t>(n));
-    std::vector<Item> bands, ckpts;
+    std::vector<Item> bands, ckpts, longs;
     for (int64_t k = 0; k < n; k++) {
         int L = 64, R = Rs.back();
         double serial = 0.0;
@@ -541,6 +557,15 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
             items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0, false};
             continue;
         }
+        if (!fit && long_ok && serial > crit) {
+            int nl = 0;
+            const double sl = long_serial(ls[k], lt[k], &nl);
+            if (nl >= 2 && sl < serial) {
+                longs.push_back(Item{k, WAVE, LONG_R, nl, lt[k], sl, false});
+                items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0, false};
+                continue;
+            }
+        }
         if (!fit && band_ok && serial > crit) {
             int nb = 0;
             const int bR = band_R(job.form, job.tr);
@@ -553,7 +578,7 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         }
         items[size_t(k)] = Item{k, L, R, nst, lt[k], 0.0, false};
     }
-    if (!bands.empty() || !ckpts.empty())
+    if (!bands.empty() || !ckpts.empty() || !longs.empty())
         items.erase(std::remove_if(items.begin(), items.end(), [](const Item &it) { return it.L == 0; }),
                     items.end());
     // group identical layouts with similar shape: sort by R, L, strips, len_t
@@ -618,6 +643,22 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         g.strip = -1;
         g.cost = int64_t(double(it.nst) * double(it.lt + BAND_L - 1) * (STEP_OVERHEAD + BAND_R));
         gs.push_back(GroupR{g, job.fast ? 1000 + BAND_R : BAND_R});
+        plan.probs.push_back(pr);
+    }
+    // long-mode problems: no groups; pa_long_kernel takes their bands in
+    // order; listed first among the band problems (their strips' partials
+    // are folded like band-mode strips')
+    std::stable_sort(longs.begin(), longs.end(), [](const Item &a, const Item &b) { return a.band > b.band; });
+    for (const Item &it : longs) {
+        Prob pr{};
+        pr.out = int32_t(it.k);
+        pr.L = WAVE;
+        pr.R = LONG_R;
+        pr.band = int32_t(plan.band_slots);
+        plan.band_slots += it.nst;
+        plan.band_probs.push_back(int32_t(plan.probs.size()));
+        plan.long_bands += (it.nst + LONG_WAVES - 1) / LONG_WAVES;
+        plan.n_long++;
         plan.probs.push_back(pr);
     }
     // band-mode problems: one group per strip, strips in order, run by the
@@ -983,6 +1024,12 @@ void run_wsb(pa_solver &s, const Input &in, const Job &job, std::unique_ptr<pa_r
     *result = std::move(r);
 }
 
+bool is_long(const Plan &pl, const Prob &pr) {
+    for (int32_t q = 0; q < pl.n_long; q++)
+        if (&pl.probs[size_t(pl.band_probs[size_t(q)])] == &pr) return true;
+    return false;
+}
+
 // A plan's problems get their lengths and offsets: payload, traceback
 // words, strip-boundary row buffer and cell values.
 void place_plan(Plan &pl, const Input &in, int bits, bool keep_vals, int val_layers) {
@@ -1005,7 +1052,7 @@ void place_plan(Plan &pl, const Input &in, int bits, bool keep_vals, int val_lay
             pr.tb_off = pl.tb_words;
             pl.tb_words += nst * (pr.len_t + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
         }
-        if (nst > 1) {
+        if (nst > 1 && !is_long(pl, pr)) {
             pr.rb_off = pl.rowbuf;
             pl.rowbuf += pr.band >= 0 ? (nst - 1) * int64_t(pr.len_t) : int64_t(pr.len_t);
         }
@@ -1015,6 +1062,18 @@ void place_plan(Plan &pl, const Input &in, int bits, bool keep_vals, int val_lay
         }
         pl.cells += double(pr.len_s) * double(pr.len_t);
     }
+    // long-mode problems hand each band's bottom row to the band below as
+    // two tagged 8-byte words per column (two float2 slots), in one range
+    // that is cleared before every launch
+    pl.xbuf_begin = pl.rowbuf;
+    for (int32_t q = 0; q < pl.n_long; q++) {
+        Prob &pr = pl.probs[size_t(pl.band_probs[size_t(q)])];
+        const int64_t nst = (pr.len_s + int64_t(WAVE) * LONG_R - 1) / (int64_t(WAVE) * LONG_R);
+        const int64_t nb = (nst + LONG_WAVES - 1) / LONG_WAVES;
+        pr.rb_off = pl.rowbuf;
+        pl.rowbuf += 2 * std::max<int64_t>(nb - 1, 0) * pr.len_t;
+    }
+    pl.xbuf_end = pl.rowbuf;
 }
 
 // The DP launches of a plan: one spec (kernel, strip height, grid, LDS) and
@@ -1167,7 +1226,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         pc.valid = false;
         pc.plan = in.schedule == PA_SCHED_PLAIN ? plan_plain(job, in.len_s, in.len_t, n)
                                                 : plan_auto(job, in.len_s, in.len_t, n, s.cus * 4,
-                                                            n_ckpt ? ckpt.data() : nullptr);
+                                                            n_ckpt ? ckpt.data() : nullptr, true, !keep_vals);
         place_plan(pc.plan, in, bits, keep_vals, val_layers);
         pc.uploaded = false;
         pc.key = key;
@@ -1210,7 +1269,8 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     const Plan &plan = pc.plan;
 
     // ---- device buffers ----------------------------------------------------
-    s.b_queue.ensure(std::max<size_t>(plan.launches.size(), 1) * 4);
+    // queue tickets: one per launch, then the long-mode band tickets
+    s.b_queue.ensure((plan.launches.size() + 1) * 4);
     const void *codes_s = nullptr, *codes_t = nullptr;
     const float *dense = nullptr;
     stage_payload(s, in, st, codes_s, codes_t, dense);
@@ -1270,12 +1330,27 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     std::vector<KArgs> largs;
     build_launches(s, job, in, plan, a, s.b_queue.as<uint32_t>(), specs, largs);
     const int n_band = int(plan.band_probs.size());
-    // the whole sequence: queue tickets and band counters cleared by kernels
-    // (no memset nodes), the DP launches, the band fold
+    LongArgs la{};
+    int long_grid = 0;
+    if (plan.n_long > 0) {
+        la.idx = s.b_band_probs.as<int32_t>();
+        la.n = plan.n_long;
+        la.total_bands = plan.long_bands;
+        la.ticket = s.b_queue.as<uint32_t>() + plan.launches.size();
+        long_grid = std::max(1, std::min(plan.long_bands, s.cus));
+    }
+    // the whole sequence: queue tickets, band counters and long-mode
+    // hand-off tags cleared by kernels (no memset nodes), the long-mode
+    // launch (the critical path), the DP launches, the band fold
     auto enqueue = [&]() -> hipError_t {
-        hipError_t e = launch_zero_u32(s.b_queue.as<uint32_t>(), int64_t(plan.launches.size()), st);
+        hipError_t e = launch_zero_u32(s.b_queue.as<uint32_t>(), int64_t(plan.launches.size()) + 1, st);
         if (e == hipSuccess && plan.band_slots > 0) e = launch_zero_u32(s.b_progress.as<uint32_t>(), plan.band_slots, st);
         if (e == hipSuccess && plan.band_slots > 0) e = launch_zero_u32(s.b_band_err.as<uint32_t>(), 1, st);
+        if (e == hipSuccess && plan.xbuf_end > plan.xbuf_begin)
+            e = launch_zero_u32(reinterpret_cast<uint32_t *>(s.b_rowbuf.as<float2>() + plan.xbuf_begin),
+                                2 * (plan.xbuf_end - plan.xbuf_begin), st);
+        if (e == hipSuccess && plan.n_long > 0)
+            e = launch_long(job.loc, job.dir, job.gap, job.form, job.tr, a, la, long_grid, st);
         for (size_t q = 0; q < specs.size() && e == hipSuccess; q++) e = launch_solve(specs[q], largs[q], st);
         if (e == hipSuccess && n_band > 0)
             e = launch_band_combine(job.loc, job.dir, job.tr, s.b_probs.as<Prob>(), s.b_band_probs.as<int32_t>(),
@@ -1287,9 +1362,11 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     // the plan and inputs uploaded just before, so a replay sees new data.
     std::string gkey(reinterpret_cast<const char *>(specs.data()), specs.size() * sizeof(LaunchSpec));
     gkey.append(reinterpret_cast<const char *>(largs.data()), largs.size() * sizeof(KArgs));
-    const int64_t extra[6] = {int64_t(plan.launches.size()), plan.band_slots, n_band,
-                              int64_t(reinterpret_cast<intptr_t>(scores)), int64_t(reinterpret_cast<intptr_t>(seeds)),
-                              int64_t(job.loc * 100 + job.dir * 10 + job.tr)};
+    const int64_t extra[10] = {int64_t(plan.launches.size()), plan.band_slots, n_band,
+                               int64_t(reinterpret_cast<intptr_t>(scores)), int64_t(reinterpret_cast<intptr_t>(seeds)),
+                               int64_t(job.loc * 100 + job.dir * 10 + job.tr), plan.n_long, plan.long_bands,
+                               plan.xbuf_end - plan.xbuf_begin,
+                               int64_t(reinterpret_cast<intptr_t>(s.b_band_probs.p))};
     gkey.append(reinterpret_cast<const char *>(extra), sizeof(extra));
 
     HIPCHK(hipEventRecord(s.ev0, st));

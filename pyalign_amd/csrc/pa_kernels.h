@@ -185,7 +185,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         float botD = DlA[R - 1], botP = W;  // this lane's last row, handed down
         // diagonal predecessor of the strip's first row at j = 0: D(row0-1, -1)
         float prevD = W;
-        if (l == 0) prevD = (row0 == 0) ? a.border_t[0] : (act ? a.border_s[row0] : W);
+        if (l == 0) prevD = (row0 == 0) ? a.border_t[0] : (act && row0 <= ls ? a.border_s[row0] : W);
 
         // The column's inputs (symbol, series row or similarity column) and,
         // for lane 0, the row above the strip (border or row buffer) are
@@ -1019,3 +1019,5 @@ static hipError_t dispatch_lg(const LaunchSpec &s, const KArgs &a, hipStream_t s
 }
 
 }  // namespace pa
+
+#include "pa_long.h"

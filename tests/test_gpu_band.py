@@ -1,6 +1,7 @@
 """Band mode on the GPU: the strips of a long problem computed concurrently
 by several wavefronts, the bottom row of each strip handed to the next
-through the row buffer and a published column count.  Results must equal
+through the row buffer and a published column count (and, for the forms it
+serves, the long mode that replaces it: tests/test_gpu_long.py).  Results must equal
 the oracle's exactly, as in every other schedule; each test also checks
 that the schedule really ran problems in band mode.
 """
@@ -54,11 +55,18 @@ def dense_of(form, k, cs, ct, os_, ot, ls, lt, table=None, eq=None, ne=None):
     return np.where(a[:, None] == b[None, :], eq, ne).astype(np.float32)
 
 
+@pytest.mark.parametrize("mode", ["long", "band"])
 @pytest.mark.parametrize("loc", list(LOCS))
 @pytest.mark.parametrize("gap", [("linear", 1), ("affine", 5, 1)], ids=str)
 @pytest.mark.parametrize("direction", ["maximize", "minimize"])
 @pytest.mark.parametrize("form", ["indexed", "binary"])
-def test_band_scores_and_paths(loc, gap, direction, form):
+def test_band_scores_and_paths(loc, gap, direction, form, mode, monkeypatch):
+    """mode "long": critical-path problems in the long mode (workgroup
+    sweeps, the default where the form allows it); "band": band mode
+    (strips on separate wavefronts), which the long mode replaces for these
+    forms."""
+    if mode == "band":
+        monkeypatch.setenv("PYALIGN_AMD_NO_LONG_MODE", "1")
     rng = np.random.default_rng(zlib.crc32(repr((loc, gap, direction, form, "band")).encode()))
     ls, lt = lengths(rng, 3, 1100, 1700, 300, 700)
     K = 4 if form == "binary" else 6
@@ -79,8 +87,13 @@ def test_band_scores_and_paths(loc, gap, direction, form):
         res = solver.solve_ragged(kind, form, ls, lt, **payload)
         # local maximisation over a table, score only, runs the fast kernel,
         # whose tall strips (32 rows a lane) beat band mode at these shapes
+        # (not the long mode)
         fast = loc == "local" and direction == "maximize" and form == "indexed" and kind == "score"
-        assert solver.last_schedule()[1] == (0 if fast else 3), solver.last_schedule()
+        groups, band, _ = solver.last_schedule()
+        if mode == "band":
+            assert band == (0 if fast else 3), solver.last_schedule()
+        else:  # long mode (also for the fast score-only job): no work groups of their own
+            assert band == 3 and groups == 0, solver.last_schedule()
         for k in range(3):
             S = dense_of(form, k, cs, ct, os_, ot, ls, lt, table, eq, ne)
             ref = O.OracleResult(O.make_config(locality=LOCS[loc][1], direction=odir, gap=gap, goal=goal,
