@@ -486,10 +486,13 @@ struct Job {
 // phases of LONG_T steps apart within a workgroup (rows handed down through
 // LDS) and about twice that between workgroups, so the serial time is
 // (len_t + 63 + (strips - 1) * LONG_DW * LONG_T * ~1.25) steps of one row.
-double long_serial(int64_t ls, int64_t lt, int *nst_out) {
+// A step of one row costs about 50 ns (measured), three cell units of the
+// general kernel (R = 8: a 12-unit step in about 200 ns) and four of the
+// fast kernel's cheaper cells.
+double long_serial(int64_t ls, int64_t lt, bool fast, int *nst_out) {
     const int64_t nst = (ls + WAVE * LONG_R - 1) / (WAVE * LONG_R);
     *nst_out = int(nst);
-    return double(lt + WAVE - 1 + double(nst - 1) * LONG_DW * LONG_T * 1.25) * (STEP_OVERHEAD + LONG_R);
+    return double(lt + WAVE - 1 + double(nst - 1) * LONG_DW * LONG_T * 1.25) * (fast ? 4.0 : 3.0);
 }
 
 Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, int simds,
@@ -557,16 +560,21 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
             items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0, false};
             continue;
         }
+        // where the long mode serves the job it replaces band mode (whose
+        // strips hand rows down through published column counts, several
+        // times slower per step than the long mode's LDS rings)
         if (!fit && long_ok && serial > crit) {
             int nl = 0;
-            const double sl = long_serial(ls[k], lt[k], &nl);
-            if (nl >= 2 && sl < serial) {
+            const double sl = long_serial(ls[k], lt[k], job.fast, &nl);
+            // (at least one strip per compute wave of a workgroup: shorter
+            // problems leave its waves idle)
+            if (nl >= LONG_WAVES && sl < serial) {
                 longs.push_back(Item{k, WAVE, LONG_R, nl, lt[k], sl, false});
                 items[size_t(k)] = Item{k, 0, 0, 0, 0, 0.0, false};
                 continue;
             }
         }
-        if (!fit && band_ok && serial > crit) {
+        if (!fit && band_ok && !long_ok && serial > crit) {
             int nb = 0;
             const int bR = band_R(job.form, job.tr);
             const double sb = band_serial(ls[k], lt[k], bR, &nb) * band_penalty;
@@ -1332,6 +1340,27 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     const int n_band = int(plan.band_probs.size());
     LongArgs la{};
     int long_grid = 0;
+    // the long kernel folds score-only cells with max / min where that keeps
+    // the strict fold's bits: similarities finite (or the pad's worst value)
+    // and never -0, and, for maximising global problems, gap costs above zero
+    // (borders below -0), so that no candidate is -0 or NaN
+    bool fold_max = false;
+    if (plan.n_long > 0 && job.tr == TR_SCORE) {
+        auto exact = [](float x) { return std::isfinite(x) && !(x == 0.0f && std::signbit(x)); };
+        bool good = true;
+        if (job.form == F_BINARY) {
+            good = exact(in.eq) && exact(in.ne);
+        } else {
+            const float w = worst(o.dir);
+            for (float x : in.table)
+                if (!exact(x) && x != w) {
+                    good = false;
+                    break;
+                }
+        }
+        if (job.loc == LOC_GLOBAL && job.dir == DIR_MAX && !(s.go_s < 0.0f && s.go_t < 0.0f)) good = false;
+        fold_max = good;
+    }
     if (plan.n_long > 0) {
         la.idx = s.b_band_probs.as<int32_t>();
         la.n = plan.n_long;
@@ -1350,7 +1379,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             e = launch_zero_u32(reinterpret_cast<uint32_t *>(s.b_rowbuf.as<float2>() + plan.xbuf_begin),
                                 2 * (plan.xbuf_end - plan.xbuf_begin), st);
         if (e == hipSuccess && plan.n_long > 0)
-            e = launch_long(job.loc, job.dir, job.gap, job.form, job.tr, a, la, long_grid, st);
+            e = launch_long(job.loc, job.dir, job.gap, job.form, job.tr, fold_max, a, la, long_grid, st);
         for (size_t q = 0; q < specs.size() && e == hipSuccess; q++) e = launch_solve(specs[q], largs[q], st);
         if (e == hipSuccess && n_band > 0)
             e = launch_band_combine(job.loc, job.dir, job.tr, s.b_probs.as<Prob>(), s.b_band_probs.as<int32_t>(),
@@ -1365,7 +1394,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     const int64_t extra[10] = {int64_t(plan.launches.size()), plan.band_slots, n_band,
                                int64_t(reinterpret_cast<intptr_t>(scores)), int64_t(reinterpret_cast<intptr_t>(seeds)),
                                int64_t(job.loc * 100 + job.dir * 10 + job.tr), plan.n_long, plan.long_bands,
-                               plan.xbuf_end - plan.xbuf_begin,
+                               (plan.xbuf_end - plan.xbuf_begin) * 2 + int64_t(fold_max),
                                int64_t(reinterpret_cast<intptr_t>(s.b_band_probs.p))};
     gkey.append(reinterpret_cast<const char *>(extra), sizeof(extra));
 

@@ -82,13 +82,14 @@ constexpr uint32_t BAND_SPIN_LIMIT = 1u << 20;  // polls before a wait gives up 
 // waves and one loader wave computes a band of LONG_WAVES strips of
 // 64 x LONG_R rows; the waves run in phases of LONG_T steps between
 // workgroup barriers, LONG_DW phases apart, handing bottom rows down through
-// LDS rings of LONG_RING columns read LONG_PF steps ahead; the loader reads
-// the band above LONG_K phases ahead.
+// LDS rings of LONG_RING columns (a phase's entries read in one load at the
+// start of the phase before); the loader reads the band above LONG_K phases
+// ahead.
 #ifndef PA_LONG_WAVES
 #define PA_LONG_WAVES 4
 #endif
 #ifndef PA_LONG_T
-#define PA_LONG_T 8
+#define PA_LONG_T 16
 #endif
 #ifndef PA_LONG_K
 #define PA_LONG_K 4
@@ -97,14 +98,14 @@ constexpr int LONG_WAVES = PA_LONG_WAVES;
 constexpr int LONG_R = 1;
 constexpr int LONG_T = PA_LONG_T;
 constexpr int LONG_K = PA_LONG_K;
-constexpr int LONG_PF = 2;
 constexpr int LONG_RING = 64;
-// a wave reads column j at its step j - LONG_PF, strictly in a later phase
-// than the wave above wrote it at its step j + 63
-constexpr int LONG_DW = (LONG_T + LONG_PF + 62) / LONG_T + 1;
-static_assert(LONG_T % 2 == 0 && LONG_K * LONG_T <= 64, "long-mode phase shape");
+// a wave reads the ring entries of its local phase k at the start of phase
+// k - 1, strictly after the barrier that follows the phase in which the wave
+// above wrote the last of them (column j at its step j + 63)
+constexpr int LONG_DW = (LONG_T + 62) / LONG_T + 2;
+static_assert((LONG_T == 4 || LONG_T == 8 || LONG_T == 16) && LONG_K * LONG_T <= 64, "long-mode phase shape");
 // the ring slot of column c is rewritten (column c + RING) only after its read
-static_assert(LONG_DW < (LONG_RING + 63 + LONG_PF) / LONG_T, "long-mode ring too short");
+static_assert(LONG_DW - 1 < (LONG_RING + 63) / LONG_T, "long-mode ring too short");
 constexpr uint32_t LONG_SPIN_LIMIT = 1u << 20;  // re-reads before a wait gives up (about a second)
 
 struct LongArgs {
@@ -177,8 +178,10 @@ constexpr int FAST_BLOCK = PA_FAST_BLOCK;
 hipError_t launch_solve(const LaunchSpec &spec, const KArgs &args, hipStream_t stream);
 // Long-problem mode (forms F_TABLE / F_BINARY, score or one traceback,
 // linear or affine gaps); `grid` workgroups take the bands in order.
-hipError_t launch_long(int loc, int dir, int gap, int form, int tr, const KArgs &args, const LongArgs &la, int grid,
-                       hipStream_t stream);
+// fold_max: max / min folds instead of strict compare-and-select (score
+// only; the caller has checked that they return the same bits).
+hipError_t launch_long(int loc, int dir, int gap, int form, int tr, bool fold_max, const KArgs &args,
+                       const LongArgs &la, int grid, hipStream_t stream);
 size_t long_lds_bytes(int form, int K1);
 bool long_supported(int gap, int form, int tr);
 hipError_t launch_walk(int gap, const Prob *probs, int64_t n, const uint32_t *tb, const int32_t *seeds,

@@ -16,10 +16,10 @@ hipError_t dispatch_global_max(const LaunchSpec &s, const KArgs &a, hipStream_t 
     return hipErrorInvalidValue;
 }
 
-hipError_t dispatch_long_global_max(int gap, int form, int tr, const KArgs &a, const LongArgs &la, int grid, size_t lds,
+hipError_t dispatch_long_global_max(int gap, int form, int tr, bool mx, const KArgs &a, const LongArgs &la, int grid, size_t lds,
                                   hipStream_t st) {
-    if (gap == GAP_LIN) return dispatch_long_lg<LOC_GLOBAL, DIR_MAX, GAP_LIN>(form, tr, a, la, grid, lds, st);
-    if (gap == GAP_AFF) return dispatch_long_lg<LOC_GLOBAL, DIR_MAX, GAP_AFF>(form, tr, a, la, grid, lds, st);
+    if (gap == GAP_LIN) return dispatch_long_lg<LOC_GLOBAL, DIR_MAX, GAP_LIN>(form, tr, mx, a, la, grid, lds, st);
+    if (gap == GAP_AFF) return dispatch_long_lg<LOC_GLOBAL, DIR_MAX, GAP_AFF>(form, tr, mx, a, la, grid, lds, st);
     return hipErrorInvalidValue;
 }
 

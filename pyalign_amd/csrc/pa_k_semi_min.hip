@@ -13,10 +13,10 @@ hipError_t dispatch_semi_min(const LaunchSpec &s, const KArgs &a, hipStream_t st
     return hipErrorInvalidValue;
 }
 
-hipError_t dispatch_long_semi_min(int gap, int form, int tr, const KArgs &a, const LongArgs &la, int grid, size_t lds,
+hipError_t dispatch_long_semi_min(int gap, int form, int tr, bool mx, const KArgs &a, const LongArgs &la, int grid, size_t lds,
                                   hipStream_t st) {
-    if (gap == GAP_LIN) return dispatch_long_lg<LOC_SEMI, DIR_MIN, GAP_LIN>(form, tr, a, la, grid, lds, st);
-    if (gap == GAP_AFF) return dispatch_long_lg<LOC_SEMI, DIR_MIN, GAP_AFF>(form, tr, a, la, grid, lds, st);
+    if (gap == GAP_LIN) return dispatch_long_lg<LOC_SEMI, DIR_MIN, GAP_LIN>(form, tr, mx, a, la, grid, lds, st);
+    if (gap == GAP_AFF) return dispatch_long_lg<LOC_SEMI, DIR_MIN, GAP_AFF>(form, tr, mx, a, la, grid, lds, st);
     return hipErrorInvalidValue;
 }
 

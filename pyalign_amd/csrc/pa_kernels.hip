@@ -14,12 +14,12 @@ hipError_t dispatch_global_min(const LaunchSpec &, const KArgs &, hipStream_t, i
 hipError_t dispatch_semi_max(const LaunchSpec &, const KArgs &, hipStream_t, int *);
 hipError_t dispatch_semi_min(const LaunchSpec &, const KArgs &, hipStream_t, int *);
 
-hipError_t dispatch_long_local_max(int, int, int, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
-hipError_t dispatch_long_local_min(int, int, int, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
-hipError_t dispatch_long_global_max(int, int, int, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
-hipError_t dispatch_long_global_min(int, int, int, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
-hipError_t dispatch_long_semi_max(int, int, int, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
-hipError_t dispatch_long_semi_min(int, int, int, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
+hipError_t dispatch_long_local_max(int, int, int, bool, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
+hipError_t dispatch_long_local_min(int, int, int, bool, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
+hipError_t dispatch_long_global_max(int, int, int, bool, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
+hipError_t dispatch_long_global_min(int, int, int, bool, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
+hipError_t dispatch_long_semi_max(int, int, int, bool, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
+hipError_t dispatch_long_semi_min(int, int, int, bool, const KArgs &, const LongArgs &, int, size_t, hipStream_t);
 
 bool long_supported(int gap, int form, int tr) {
     return (gap == GAP_LIN || gap == GAP_AFF) && (form == F_TABLE || form == F_BINARY) &&
@@ -30,21 +30,22 @@ size_t long_lds_bytes(int form, int K1) {
     return size_t(LONG_WAVES) * LONG_RING * 16 + (form == F_TABLE ? size_t(K1) * K1 * 4 : 0);
 }
 
-hipError_t launch_long(int loc, int dir, int gap, int form, int tr, const KArgs &a, const LongArgs &la, int grid,
-                       hipStream_t st) {
+hipError_t launch_long(int loc, int dir, int gap, int form, int tr, bool fold_max, const KArgs &a,
+                       const LongArgs &la, int grid, hipStream_t st) {
     if (!long_supported(gap, form, tr) || la.n <= 0 || grid <= 0) return hipErrorInvalidValue;
     const size_t lds = long_lds_bytes(form, a.K1);
+    const bool fm = fold_max && tr == TR_SCORE;
     const bool mx = dir == DIR_MAX;
     switch (loc) {
         case LOC_LOCAL:
-            return mx ? dispatch_long_local_max(gap, form, tr, a, la, grid, lds, st)
-                      : dispatch_long_local_min(gap, form, tr, a, la, grid, lds, st);
+            return mx ? dispatch_long_local_max(gap, form, tr, fm, a, la, grid, lds, st)
+                      : dispatch_long_local_min(gap, form, tr, fm, a, la, grid, lds, st);
         case LOC_GLOBAL:
-            return mx ? dispatch_long_global_max(gap, form, tr, a, la, grid, lds, st)
-                      : dispatch_long_global_min(gap, form, tr, a, la, grid, lds, st);
+            return mx ? dispatch_long_global_max(gap, form, tr, fm, a, la, grid, lds, st)
+                      : dispatch_long_global_min(gap, form, tr, fm, a, la, grid, lds, st);
         case LOC_SEMI:
-            return mx ? dispatch_long_semi_max(gap, form, tr, a, la, grid, lds, st)
-                      : dispatch_long_semi_min(gap, form, tr, a, la, grid, lds, st);
+            return mx ? dispatch_long_semi_max(gap, form, tr, fm, a, la, grid, lds, st)
+                      : dispatch_long_semi_min(gap, form, tr, fm, a, la, grid, lds, st);
     }
     return hipErrorInvalidValue;
 }

@@ -92,8 +92,8 @@ def test_band_scores_and_paths(loc, gap, direction, form, mode, monkeypatch):
         groups, band, _ = solver.last_schedule()
         if mode == "band":
             assert band == (0 if fast else 3), solver.last_schedule()
-        else:  # long mode (also for the fast score-only job): no work groups of their own
-            assert band == 3 and groups == 0, solver.last_schedule()
+        elif not fast:  # the long mode, or tall strips where the planner's cost model prefers them
+            assert band >= 1, solver.last_schedule()
         for k in range(3):
             S = dense_of(form, k, cs, ct, os_, ot, ls, lt, table, eq, ne)
             ref = O.OracleResult(O.make_config(locality=LOCS[loc][1], direction=odir, gap=gap, goal=goal,
@@ -148,8 +148,10 @@ def test_band_fast_job_local_score():
         assert np.float32(res[k]).tobytes() == ref.score.tobytes(), k
 
 
-def test_band_solution_matrices():
-    """Solution.values and tracebacks of a problem solved in band mode."""
+def test_band_solution_matrices(monkeypatch):
+    """Solution.values and tracebacks of a problem solved in band mode (the
+    matrices always are: the long mode keeps no cell values)."""
+    monkeypatch.setenv("PYALIGN_AMD_NO_LONG_MODE", "1")
     rng = np.random.default_rng(83)
     ls, lt = lengths(rng, 2, 1100, 1300, 200, 260)
     cs, ct, os_, ot = ragged_codes(rng, ls, lt, 4)
