@@ -482,17 +482,17 @@ struct Job {
 // wavefront sweeps its strips of BAND_L x BAND_R rows in order and keeps
 // every strip's bottom row, from which the backward recomputes it.
 // Long mode (pa_long_kernel) for a problem that would otherwise be the
-// critical path: its strips of 64 rows on the waves of workgroups, LONG_DW
-// phases of LONG_T steps apart within a workgroup (rows handed down through
-// LDS) and about twice that between workgroups, so the serial time is
-// (len_t + 63 + (strips - 1) * LONG_DW * LONG_T * ~1.25) steps of one row.
+// critical path: its strips of 64 rows on the waves of workgroups,
+// long_dw phases of LONG_T steps apart within a workgroup (rows handed down
+// through LDS) and a little more between workgroups, so the serial time is
+// (len_t + 63 + (strips - 1) * long_dw * LONG_T * ~1.25) steps of one row.
 // A step of one row costs about 50 ns (measured), three cell units of the
 // general kernel (R = 8: a 12-unit step in about 200 ns) and four of the
 // fast kernel's cheaper cells.
-double long_serial(int64_t ls, int64_t lt, bool fast, int *nst_out) {
+double long_serial(int64_t ls, int64_t lt, int form, bool fast, int *nst_out) {
     const int64_t nst = (ls + WAVE * LONG_R - 1) / (WAVE * LONG_R);
     *nst_out = int(nst);
-    return double(lt + WAVE - 1 + double(nst - 1) * LONG_DW * LONG_T * 1.25) * (fast ? 4.0 : 3.0);
+    return double(lt + WAVE - 1 + double(nst - 1) * long_dw(form) * LONG_T * 1.25) * (fast ? 4.0 : 3.0);
 }
 
 Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, int simds,
@@ -565,7 +565,7 @@ Plan plan_auto(const Job &job, const int32_t *ls, const int32_t *lt, int64_t n, 
         // times slower per step than the long mode's LDS rings)
         if (!fit && long_ok && serial > crit) {
             int nl = 0;
-            const double sl = long_serial(ls[k], lt[k], job.fast, &nl);
+            const double sl = long_serial(ls[k], lt[k], job.form, job.fast, &nl);
             // (at least one strip per compute wave of a workgroup: shorter
             // problems leave its waves idle)
             if (nl >= LONG_WAVES && sl < serial) {

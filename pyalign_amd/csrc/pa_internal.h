@@ -81,10 +81,9 @@ constexpr uint32_t BAND_SPIN_LIMIT = 1u << 20;  // polls before a wait gives up 
 // Long-problem mode (pa_long_kernel): a workgroup of LONG_WAVES compute
 // waves and one loader wave computes a band of LONG_WAVES strips of
 // 64 x LONG_R rows; the waves run in phases of LONG_T steps between
-// workgroup barriers, LONG_DW phases apart, handing bottom rows down through
-// LDS rings of LONG_RING columns (a phase's entries read in one load at the
-// start of the phase before); the loader reads the band above LONG_K phases
-// ahead.
+// workgroup barriers, long_dw phases apart, handing bottom rows down through
+// LDS rings of LONG_RING columns; the loader reads the band above LONG_K
+// phases ahead and publishes the band's bottom row to the band below.
 #ifndef PA_LONG_WAVES
 #define PA_LONG_WAVES 4
 #endif
@@ -99,13 +98,18 @@ constexpr int LONG_R = 1;
 constexpr int LONG_T = PA_LONG_T;
 constexpr int LONG_K = PA_LONG_K;
 constexpr int LONG_RING = 64;
-// a wave reads the ring entries of its local phase k at the start of phase
-// k - 1, strictly after the barrier that follows the phase in which the wave
-// above wrote the last of them (column j at its step j + 63)
-constexpr int LONG_DW = (LONG_T + 62) / LONG_T + 2;
-static_assert((LONG_T == 4 || LONG_T == 8 || LONG_T == 16) && LONG_K * LONG_T <= 64, "long-mode phase shape");
-// the ring slot of column c is rewritten (column c + RING) only after its read
-static_assert(LONG_DW - 1 < (LONG_RING + 63) / LONG_T, "long-mode ring too short");
+// Wave w runs b_w = -w mod LONG_T steps behind the phase grid, so that the
+// LONG_T columns its lane 63 finishes in a phase and the LONG_T columns the
+// next wave's lane 0 takes in a phase each fill one run of ring slots.  A
+// wave reads its local phase k's entries at the start of phase k (the table
+// form also the first of phase k + 1); the wave above wrote them in its
+// local phase k + 64 / LONG_T (k + 64 / LONG_T - 1 above wave 1), which must
+// end before that read: long_dw >= 64 / LONG_T + 1 (+ 1 for the table
+// form).  It rewrites their slots 64 / LONG_T phases later, which must come
+// after the read: long_dw < 128 / LONG_T - 1.
+constexpr int long_dw(int form) { return 64 / LONG_T + (form == F_TABLE ? 2 : 1); }
+static_assert((LONG_T == 8 || LONG_T == 16) && LONG_K * LONG_T <= 64, "long-mode phase shape");
+static_assert(long_dw(F_TABLE) < 128 / LONG_T - 1 && LONG_WAVES <= LONG_T, "long-mode ring timing");
 constexpr uint32_t LONG_SPIN_LIMIT = 1u << 20;  // re-reads before a wait gives up (about a second)
 
 struct LongArgs {

@@ -13,25 +13,37 @@
 // Hand-offs.  Inside a strip, lane l's bottom row (D, P) and the column
 // symbol reach lane l+1 one step later through a DPP wave shift
 // (wave_shr:1); lane 0's inputs -- the row above the strip and the column's
-// symbol -- come from an LDS ring filled by the wave above (its lane 63
-// writes (D, P, symbol) of each column it finishes).  The waves of a
-// workgroup run in phases of LONG_T steps separated by workgroup barriers;
-// wave w runs LONG_DW phases behind wave w-1, so every ring entry a wave
-// reads was written in an earlier phase (the barrier orders the LDS write
-// before the read).  Wave 0 fills its own ring from the strip border (the
-// first band) or from the band above: that band's last wave stores each
-// column's bottom (D, P) to global memory as two 8-byte words tagged with
-// the column (relaxed atomics at device scope: no cache flush), and wave 0
-// reads them LONG_K phases before they are due, re-reading (bounded) any
-// whose tag is not there yet.  The tagged buffer is cleared before the
-// launch.
+// symbol -- arrive as the shift's fill value, read from an LDS ring filled by
+// the wave above (its lane 63 writes (D, P, symbol) of each column it
+// finishes).  The waves of a workgroup run in phases of LONG_T steps
+// separated by workgroup barriers; wave w runs long_dw phases behind wave
+// w-1, so every ring entry a wave reads was written in an earlier phase (the
+// barrier orders the LDS write before the read).  Lane 0 alone reads a
+// phase's ring entries, at the start of the phase (a read by every lane
+// would move 64 copies of each through the CU's LDS, which the four waves
+// share), so each of its inputs is one DPP move whose fill operand is
+// already in a register; lane 63's outputs pass through a shift register
+// (lane 0 <- lane 63 by a wave rotate, the rest shifted down a lane) and
+// lanes 0 .. LONG_T - 1 write the phase's entries at its end.  Wave 0's
+// ring holds the strip border (the first band) or the band above's bottom
+// row; the band's last wave writes its bottom row to one more ring, which
+// the loader wave publishes to global memory as two 8-byte words per column
+// tagged with the column (relaxed atomics at device scope: no cache flush).
+// The loader of the band below reads them LONG_K phases before they are
+// due, re-reading (bounded) any whose tag is not there yet.  The tagged
+// buffer is cleared before the launch.
 //
 // Cells.  The same fold as the general kernel in the reference's push order
 // with strict improvement (pyalign/algorithm/solver.h:1311-1427; linear
 // 2837-2854, Gotoh 2989-3035), so values, traceback bits (the general
 // kernel's layout: the walk kernel reads them unchanged) and seeds are
-// identical.  Each strip leaves its part of the seed search in a band-mode
-// partial that pa_band_combine_kernel folds in strip order.
+// identical.  A phase in which every lane's column lies inside the problem
+// runs without branches: rows below the last one score the worst
+// similarity, so they never win a seed (scores) or are masked out of it
+// (local paths), and every lane tracks its own last-row optimum, of which
+// the last row's lane is kept.  Each strip leaves its part of the seed
+// search in a band-mode partial that pa_band_combine_kernel folds in strip
+// order.
 #pragma once
 
 #include "pa_internal.h"
@@ -45,6 +57,22 @@ __device__ __forceinline__ float shr1_in(float x, float in) {
 __device__ __forceinline__ int shr1_in_i(int x, int in) {
     return __builtin_amdgcn_update_dpp(in, x, 0x138, 0xf, 0xf, false);
 }
+// shift register step: lane k <- reg lane k-1, lane 0 <- x lane 63 (wave_ror:1)
+__device__ __forceinline__ float push63(float reg, float x) {
+    return shr1_in(reg, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x13C, 0xf, 0xf, false)));
+}
+__device__ __forceinline__ int push63_i(int reg, int x) {
+    return shr1_in_i(reg, __builtin_amdgcn_mov_dpp(x, 0x13C, 0xf, 0xf, false));
+}
+
+// the step offset of compute wave w behind the phase grid (-w mod LONG_T)
+__device__ __forceinline__ constexpr int long_boff(int w) { return (LONG_T - w % LONG_T) % LONG_T; }
+// the first ring slot of local phase k: lane 0 of wave w reads column
+// k T - b_w + t from slot (k T + t) mod RING; lane 63 of wave w writes column
+// k T - b_w + t - 63 to slot (k T + long_ooff(w) + t) mod RING of ring w + 1
+__device__ __forceinline__ constexpr int long_ooff(int w) {
+    return ((long_boff(w + 1) - long_boff(w) - 63) % LONG_RING + LONG_RING) % LONG_RING;
+}
 
 // MX: the strict compare-and-select of every candidate fold done as max
 // (maximising) or min; the host chooses it for score-only solves where it
@@ -54,21 +82,22 @@ __device__ __forceinline__ int shr1_in_i(int x, int in) {
 // value.
 template <int LOC, int DIR, int GAP, int FORM, int TR, bool MX>
 __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs a, LongArgs la) {
-    constexpr int R = LONG_R;
+    static_assert(LONG_R == 1, "one row per lane");
     constexpr int T = LONG_T;
     constexpr int K = LONG_K;
     constexpr int RING = LONG_RING;
+    constexpr int LW = LONG_WAVES;
+    constexpr int DW = long_dw(FORM);
     constexpr float W = worst<DIR>();
-    constexpr int BITS = TR == TR_ONE ? (GAP == GAP_AFF ? 4 : 2) : 0;
-    constexpr int WPC = TR == TR_ONE ? (BITS * R + 31) / 32 : 1;
     static_assert(FORM == F_TABLE || FORM == F_BINARY, "long mode reads table codes or binary symbols");
     static_assert(!MX || TR == TR_SCORE, "max folds carry no traceback bits");
 
     extern __shared__ float pa_dyn_lds[];
-    // ring[w][c % RING]: (D, P, symbol bits, -) of the row above compute wave
-    // w's strip at column c; then the similarity table (F_TABLE)
+    // ring[w] (w < LW): (D, P, symbol bits, -) of the row above compute wave
+    // w's strip; ring[LW]: the band's bottom row, for the loader to publish;
+    // then the similarity table (F_TABLE)
     float4 *ring = reinterpret_cast<float4 *>(pa_dyn_lds);
-    float *tab = pa_dyn_lds + LONG_WAVES * RING * 4;
+    float *tab = pa_dyn_lds + (LW + 1) * RING * 4;
     __shared__ int s_ticket;
     if constexpr (FORM == F_TABLE) {
         const int n = a.K1 * a.K1;
@@ -76,7 +105,7 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
     }
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int l = threadIdx.x & (WAVE - 1);
-    const bool loader = wave == LONG_WAVES;
+    const bool loader = wave == LW;
 
     while (true) {
         __syncthreads();  // the table is loaded; the last band is done with the rings
@@ -90,17 +119,18 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
         int nst = 0;
         for (;;) {
             pr = a.probs[la.idx[q]];
-            nst = (pr.len_s + WAVE * R - 1) / (WAVE * R);
-            const int nbands = (nst + LONG_WAVES - 1) / LONG_WAVES;
+            nst = (pr.len_s + WAVE - 1) / WAVE;
+            const int nbands = (nst + LW - 1) / LW;
             if (band < nbands || q + 1 >= la.n) break;
             band -= nbands;
             ++q;
         }
         const int lt = pr.len_t;
         const int nsp = lt + WAVE - 1;  // steps per strip
-        // every wave runs the same number of phases (workgroup barriers), a
-        // multiple of K (the loader's unrolled read-ahead)
-        const int phases = ((LONG_WAVES - 1) * LONG_DW + (nsp + T - 1) / T + K - 1) / K * K;
+        // every wave runs the same number of phases (workgroup barriers): up
+        // to the last wave's last step and one more (the loader publishes
+        // its entries), a multiple of K (the loader's unrolled read-ahead)
+        const int phases = ((LW - 1) * DW + (nsp + T - 1) / T + 2 + K - 1) / K * K;
         // the band above hands its bottom row down through xb (two 8-byte
         // words per column, tagged with the column + 1)
         uint64_t *xb = reinterpret_cast<uint64_t *>(a.rowbuf + pr.rb_off);
@@ -111,6 +141,24 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
             // the problem) so that no wait is placed at a branch join: the
             // words of phase p are read at phase p - K and waited for only
             // when they are staged.
+            // ---- the band's bottom row: published a phase after the last
+            // wave wrote it (its strip has a successor in the band below)
+            constexpr int wl = LW - 1;
+            const bool pub = band * LW + wl + 1 < nst;
+            uint64_t *xb_out = xb + 2 * int64_t(band) * lt;
+            const float4 *oring = ring + LW * RING;
+            auto publish = [&](int p) {  // the entries the last wave wrote in phase p - 1
+                const int k = p - 1 - wl * DW;
+                const int c = k * T - long_boff(wl) + l - (WAVE - 1);
+                if (pub && k >= 0 && l < T && c >= 0 && c < lt) {
+                    const float4 e = oring[((k * T + long_ooff(wl)) & (RING - 1)) + l];
+                    const uint64_t tag = uint64_t(c + 1) << 32;
+                    __hip_atomic_store(xb_out + 2 * c, tag | __float_as_uint(e.x), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(xb_out + 2 * c + 1, tag | __float_as_uint(e.y), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+            };
             auto run = [&](auto from_band) {
                 constexpr bool XB = decltype(from_band)::value;
                 const uint64_t *xb_in = xb + 2 * int64_t(XB ? band - 1 : 0) * lt;
@@ -159,8 +207,8 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                     }
                 };
                 // wave 0 reads phase k's columns [k T, (k + 1) T) at the start
-                // of phase k - 1 (phase 0's at its own start): lane l < T
-                // stages column due(p) = (p + 2) T + l at phase p
+                // of phase k - 1 (phases 0 and 1 at the start of phase 0):
+                // lane l < T stages column due(p) = (p + 2) T + l at phase p
                 const bool mine = l < T;
                 auto due = [&](int p) { return (p + 2) * T + (l < T ? l : T - 1); };
                 {   // phases 0 and 1: columns [0, 2 T)
@@ -179,6 +227,7 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
 #pragma unroll
                     for (int u = 0; u < K; ++u) {
                         stage(due(p0 + u), mine, hD[u], hP[u], hs[u]);
+                        publish(p0 + u);
                         issue(due(p0 + u + K), hD[u], hP[u], hs[u]);
                         __syncthreads();
                     }
@@ -189,94 +238,97 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
             continue;
         }
 
-        // ---- a compute wave: strip st ---------------------------------------
+        // ---- a compute wave: strip st, one row per lane ---------------------
         const int ls = pr.len_s;
-        const int st = band * LONG_WAVES + wave;
+        const int st = band * LW + wave;
         const bool has = st < nst;
-        const int row0 = st * WAVE * R + l * R;
+        const int row = st * WAVE + l;
+        const bool rv = has && row < ls;
         const int last = ls - 1;
-        const int st_last = last / (WAVE * R);
-        const int l_last = (last % (WAVE * R)) / R;
-        const int r_last = last % R;
-        uint64_t *xb_out = xb + 2 * int64_t(band) * lt;
-        const bool out_x = has && wave == LONG_WAVES - 1 && st + 1 < nst;
-        const float4 *my_ring = ring + wave * RING;
-        float4 *next_ring = ring + (wave + 1) * RING;
+        const int st_last = last / WAVE;
+        const int l_last = last % WAVE;
+        const int bw = long_boff(wave);
+        const int first = wave * DW;  // this wave's local phase 0
+        const int kmax = (nsp + bw + T - 1) / T;  // its phases: steps -b_w .. nsp - 1
+        const float4 *in_ring = ring + wave * RING;
+        float4 *out_ring = ring + (wave + 1) * RING;
+        const int ooff = long_ooff(wave);
         const float go_s = a.go_s, ge_s = a.ge_s, go_t = a.go_t, ge_t = a.ge_t;
         [[maybe_unused]] const int PAD = a.K1 - 1;
 
-        int rowsym[R];
-        bool rv[R];
-        float DlA[R], DlB[R], Ql[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = row0 + r;
-            rv[r] = has && i < ls;
-            if constexpr (FORM == F_TABLE) {
-                rowsym[r] = (rv[r] ? int(((const uint8_t *)a.codes_s)[pr.off_s + i]) : PAD) * a.K1;
-            } else {
-                rowsym[r] = rv[r] ? int(((const uint32_t *)a.codes_s)[pr.off_s + i]) : 0;
+        // the row's symbol; rows below the last one score the worst value
+        int rowsym;
+        float veq = W, vne = W;
+        if constexpr (FORM == F_TABLE) {
+            rowsym = (rv ? int(((const uint8_t *)a.codes_s)[pr.off_s + row]) : PAD) * a.K1;
+        } else {
+            rowsym = rv ? int(((const uint32_t *)a.codes_s)[pr.off_s + row]) : 0;
+            if (rv) {
+                veq = a.eq;
+                vne = a.ne;
             }
-            DlA[r] = rv[r] ? a.border_s[i + 1] : W;  // D(i, -1)
-            DlB[r] = DlA[r];
-            Ql[r] = W;  // Q(i, -1)
         }
-        // this lane's bottom row as the next lane sees it one step later; at
-        // the start D(row0 + R - 1, -1), the next lane's diagonal at j = 0
-        float botD = DlA[R - 1], botP = W;
-        // D(row0 - 1, -1); lanes below the last row (all rows padding) take W
-        float prevD = (row0 == 0) ? a.border_t[0] : (has && row0 <= ls ? a.border_s[row0] : W);
-        int msym = 0;                                 // the column symbol this lane used last step
-        // ring entries of this phase's and the next phase's columns, lane t
-        // holding column k T + t of local phase k (lane 0 of the strip takes
-        // its inputs from them with readlane)
-        float4 cur = make_float4(W, W, 0.0f, 0.0f), nxt = cur;
-        [[maybe_unused]] float Snext[R];             // F_TABLE: next step's similarities
-#pragma unroll
-        for (int r = 0; r < R; ++r) Snext[r] = 0.0f;
+        float Dl = rv ? a.border_s[row + 1] : W;  // D(i, j - 1), at the start D(i, -1)
+        float Ql = W;                              // Q(i, j - 1)
+        // this lane's D and P as the next lane sees them one step later
+        float botD = Dl, botP = W;
+        // D(i - 1, j - 1); at the start D(i - 1, -1); rows below the last take W
+        float prevD = (row == 0) ? a.border_t[0] : (has && row <= ls ? a.border_s[row] : W);
+        int msym = 0;  // the column symbol this lane used last step
+        [[maybe_unused]] float Snext = 0.0f;  // F_TABLE: next step's similarity
         float best = W;
-        int bi = -1, bj = -1;
+        int bj = -1;
         float rbest = W, cbest = W;
         int rj = -1, ci = -1;
         float gscore = W;
-        [[maybe_unused]] uint32_t *tbp =
-            TR == TR_ONE ? a.tb + pr.tb_off + int64_t(st) * nsp * WAVE * WPC + l * WPC : nullptr;
+        [[maybe_unused]] uint32_t *tbp = TR == TR_ONE ? a.tb + pr.tb_off + int64_t(st) * nsp * WAVE + l : nullptr;
 
-        // the fold of one direction: strict improvement keeps the first
-        // candidate on ties (the old value wins), or max / min where exact
+        // the fold of one direction: max / min where exact
         auto opt = [](float x, float y) {
             if constexpr (DIR == DIR_MAX) return fmaxf(x, y);
             else return fminf(x, y);
         };
-        // binary similarities held in registers (one select per cell)
-        float veq = a.eq, vne = a.ne;
-        asm volatile("" : "+v"(veq), "+v"(vne));
-        const bool lane_out = l == WAVE - 1;  // the lane whose bottom row goes down
-        const bool out_ring = has && wave + 1 < LONG_WAVES;
 
-        // R cells of column j: Dd is D(row0 - 1, j - 1), (Dup, Pup) the row
-        // above at j; Din holds D(i, j - 1), Dout gets D(i, j)
-        auto cells = [&](int j, float Dd, float Dup, float Pup, const float (&S)[R], float (&Din)[R], float (&Dout)[R],
-                         auto checked) -> uint32_t {
+        // one step: column j = s - l; `e` is lane 0's ring entry (column s),
+        // `en` the next step's
+        // lane 63's last T outputs (D, P, symbol), newest in lane 0
+        float oD = W, oP = W;
+        int oS = 0;
+        auto body = [&](int s, const float4 &e, const float4 &en, auto checked) {
             constexpr bool CHK = decltype(checked)::value;
-            uint32_t tbw = 0;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
+            const float inD = shr1_in(botD, e.x);
+            const float inP = shr1_in(botP, e.y);
+            const int tsym = shr1_in_i(msym, __float_as_int(e.z));
+            msym = tsym;
+            float S;
+            if constexpr (FORM == F_TABLE) {
+                if constexpr (CHK) S = tab[rowsym + int(min(uint32_t(tsym), uint32_t(PAD)))];
+                else S = Snext;
+                // the next step's symbol is the previous lane's current one
+                const int nsym = shr1_in_i(tsym, __float_as_int(en.z));
+                Snext = tab[rowsym + int(min(uint32_t(nsym), uint32_t(PAD)))];
+            } else {
+                S = rowsym == tsym ? veq : vne;
+            }
+            const float Dd = prevD;
+            prevD = inD;
+            const int j = s - l;
+            if (!CHK || (j >= 0 && j < lt)) {
                 float D, P = 0.0f, Q = 0.0f;
-                uint32_t bits = 0;
-                const float dg = Dd + S[r];
+                [[maybe_unused]] uint32_t bits = 0;
+                const float dg = Dd + S;
                 if constexpr (MX) {
                     if constexpr (GAP == GAP_LIN) {
-                        D = opt(opt(dg, Dup + go_s), Din[r] + go_t);
+                        D = opt(opt(dg, inD + go_s), Dl + go_t);
                     } else {
-                        P = opt(Dup + go_s, Pup + ge_s);
-                        Q = opt(Din[r] + go_t, Ql[r] + ge_t);
+                        P = opt(inD + go_s, inP + ge_s);
+                        Q = opt(Dl + go_t, Ql + ge_t);
                         D = opt(opt(dg, P), Q);
                     }
                     if constexpr (LOC == LOC_LOCAL) D = opt(D, 0.0f);
                 } else if constexpr (GAP == GAP_LIN) {
-                    const float up = Dup + go_s;
-                    const float lf = Din[r] + go_t;
+                    const float up = inD + go_s;
+                    const float lf = Dl + go_t;
                     int src;
                     if constexpr (LOC == LOC_LOCAL) {
                         D = 0.0f;
@@ -290,8 +342,8 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                     if (better<DIR>(lf, D)) { D = lf; src = 3; }
                     bits = uint32_t(src);
                 } else {
-                    const float po = Dup + go_s, pe = Pup + ge_s;
-                    const float qo = Din[r] + go_t, qe = Ql[r] + ge_t;
+                    const float po = inD + go_s, pe = inP + ge_s;
+                    const float qo = Dl + go_t, qe = Ql + ge_t;
                     const bool pext = better<DIR>(pe, po);
                     P = pext ? pe : po;
                     const bool qext = better<DIR>(qe, qo);
@@ -310,172 +362,89 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                     bits = uint32_t(src) | (pext ? 4u : 0u) | (qext ? 8u : 0u);
                 }
                 if constexpr (LOC == LOC_LOCAL) {
-                    const int i = row0 + r;
                     if constexpr (TR == TR_SCORE) {
                         best = MX ? opt(best, D) : (better<DIR>(D, best) ? D : best);  // padded rows stay below
-                    } else if (!CHK || rv[r]) {
-                        // Optima over i desc, j desc (solver.h:2050-2056)
-                        if (better<DIR>(D, best) || (D == best && i >= bi)) {
+                    } else {
+                        // Optima over i desc, j desc (solver.h:2050-2056): in
+                        // one row a later column wins a tie
+                        if (rv && (DIR == DIR_MAX ? D >= best : D <= best)) {
                             best = D;
-                            bi = i;
                             bj = j;
                         }
                     }
+                } else if constexpr (LOC == LOC_SEMI) {
+                    // every lane tracks its row; the last row's lane is kept
+                    // (last row, j ascending: solver.h:2285-2287)
+                    if constexpr (TR == TR_SCORE && MX) {
+                        rbest = opt(rbest, D);
+                    } else if (better<DIR>(D, rbest)) {
+                        rbest = D;
+                        rj = j;
+                    }
+                    if constexpr (CHK) {
+                        if (rv && j == lt - 1) {  // last column (solver.h:2289-2291)
+                            cbest = D;
+                            ci = row;
+                        }
+                    }
+                } else {
+                    if constexpr (CHK) {
+                        if (j == lt - 1) gscore = D;
+                    }
                 }
-                if constexpr (TR == TR_ONE) tbw |= bits << (r * BITS);
-                Dd = Din[r];
-                Dout[r] = D;
-                Ql[r] = Q;
-                Dup = D;
-                Pup = P;
+                if constexpr (TR == TR_ONE) tbp[int64_t(s) * WAVE] = bits;
+                Dl = D;
+                Ql = Q;
+                botD = D;
+                botP = P;
             }
-            botD = Dup;
-            botP = Pup;
-            return tbw;
         };
 
-        auto lane_in = [](const float4 &e, int t, int k) {  // word k of lane t's entry
-            const float x = k == 0 ? e.x : (k == 1 ? e.y : e.z);
-            return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), t));
-        };
-        // one step: column j = s - l; t = s mod T (its ring entry is lane t's of cur)
-        auto step = [&](int s, auto tc, float (&Din)[R], float (&Dout)[R], auto checked) {
+        auto step = [&](int s, const float4 &e, const float4 &en, auto checked) {
             constexpr bool CHK = decltype(checked)::value;
-            constexpr int t = decltype(tc)::value;
-            const int j = s - l;
-            const float inD = shr1_in(botD, lane_in(cur, t, 0));
-            const float inP = shr1_in(botP, lane_in(cur, t, 1));
-            const int tsym = shr1_in_i(msym, __float_as_int(lane_in(cur, t, 2)));
-            msym = tsym;
-            float S[R];
-            if constexpr (FORM == F_TABLE) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) S[r] = Snext[r];
-                // the next step's symbol is the previous lane's current one
-                const float ns = t + 1 < T ? lane_in(cur, (t + 1) % T, 2) : lane_in(nxt, 0, 2);
-                const int nsym = shr1_in_i(tsym, __float_as_int(ns));
-#pragma unroll
-                for (int r = 0; r < R; ++r) Snext[r] = tab[rowsym[r] + min(nsym, PAD)];
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; ++r) S[r] = (rowsym[r] == tsym) ? veq : vne;
-                if constexpr (CHK) {
-#pragma unroll
-                    for (int r = 0; r < R; ++r) S[r] = rv[r] ? S[r] : W;
-                }
-            }
-            const float Ddiag = prevD;
-            prevD = inD;
-            if constexpr (!CHK) {
-                // a steady step: every lane on a column before the last, every
-                // row real and no seed row (the strip is not the last)
-                [[maybe_unused]] const uint32_t tbw = cells(j, Ddiag, inD, inP, S, Din, Dout, checked);
-                if constexpr (TR == TR_ONE) tbp[int64_t(s) * WAVE * WPC] = tbw;
-                const int jo = s - (WAVE - 1);  // lane 63's column
-                if (lane_out) {
-                    if (out_ring) {
-                        next_ring[jo & (RING - 1)] = make_float4(botD, botP, __uint_as_float(uint32_t(tsym)), 0.0f);
-                    } else if (out_x) {
-                        const uint64_t tag = uint64_t(jo + 1) << 32;
-                        __hip_atomic_store(xb_out + 2 * jo, tag | __float_as_uint(botD), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                        __hip_atomic_store(xb_out + 2 * jo + 1, tag | __float_as_uint(botP), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                }
-                return;
-            }
-            if (has && j >= 0 && j < lt) {
-                [[maybe_unused]] const uint32_t tbw = cells(j, Ddiag, inD, inP, S, Din, Dout, checked);
-                if constexpr (TR == TR_ONE) tbp[int64_t(s) * WAVE * WPC] = tbw;
-                if constexpr (LOC == LOC_SEMI) {
-                    if (st == st_last && l == l_last) {
-                        float v = Dout[0];
-#pragma unroll
-                        for (int r = 1; r < R; ++r) v = (r == r_last) ? Dout[r] : v;
-                        if (better<DIR>(v, rbest)) { rbest = v; rj = j; }  // last row, j ascending (solver.h:2285-2287)
-                    }
-                    if (j == lt - 1) {
-#pragma unroll
-                        for (int r = 0; r < R; ++r)  // last column, i ascending (solver.h:2289-2291)
-                            if (rv[r] && better<DIR>(Dout[r], cbest)) { cbest = Dout[r]; ci = row0 + r; }
-                    }
-                }
-                if constexpr (LOC == LOC_GLOBAL) {
-                    if (st == st_last && l == l_last && j == lt - 1) {
-                        float v = Dout[0];
-#pragma unroll
-                        for (int r = 1; r < R; ++r) v = (r == r_last) ? Dout[r] : v;
-                        gscore = v;
-                    }
-                }
-            }
+            if (!CHK || s >= 0) body(s, e, en, checked);  // (steps before the strip's first do nothing)
             // lane 63 hands its bottom row (column s - 63) down
-            if (lane_out && j >= 0 && j < lt && has) {
-                if (out_ring) {
-                    next_ring[j & (RING - 1)] = make_float4(botD, botP, __uint_as_float(uint32_t(tsym)), 0.0f);
-                } else if (out_x) {
-                    const uint64_t tag = uint64_t(j + 1) << 32;
-                    __hip_atomic_store(xb_out + 2 * j, tag | __float_as_uint(botD), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(xb_out + 2 * j + 1, tag | __float_as_uint(botP), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
+            oD = push63(oD, botD);
+            oP = push63(oP, botP);
+            oS = push63_i(oS, msym);
         };
 
-        const int first = wave * LONG_DW;  // this wave's first phase
-        // a strip whose rows are all real and that holds no seed row skips
-        // the row and seed checks in its steady steps (every lane on a column
-        // before the last one)
-        const bool full = has && st < st_last;
-        auto ring_entries = [&](int k) { return my_ring[(k * T + (l & (T - 1))) & (RING - 1)]; };
+        // a phase's ring entries are read by lane 0 at its start (written in
+        // an earlier phase); the first step waits for the first of them
+        auto phase_steps = [&](int s0, int obase, auto checked) {
+            float4 cur[T + 1];
+            if (l == 0) {
+                const float4 *src = in_ring + ((s0 + bw) & (RING - 1));
+#pragma unroll
+                for (int t = 0; t < T; ++t) cur[t] = src[t];
+                if constexpr (FORM == F_TABLE) cur[T] = in_ring[(s0 + bw + T) & (RING - 1)];
+            }
+#define PA_LSTEP(t) \
+    if constexpr ((t) < T) step(s0 + (t), cur[(t) % T], cur[(t) + 1 <= T ? (t) + 1 : T], checked)
+            PA_LSTEP(0); PA_LSTEP(1); PA_LSTEP(2); PA_LSTEP(3); PA_LSTEP(4); PA_LSTEP(5); PA_LSTEP(6); PA_LSTEP(7);
+            PA_LSTEP(8); PA_LSTEP(9); PA_LSTEP(10); PA_LSTEP(11); PA_LSTEP(12); PA_LSTEP(13); PA_LSTEP(14);
+            PA_LSTEP(15);
+#undef PA_LSTEP
+            // lane k holds step T - 1 - k's output
+            if (l < T) out_ring[obase + T - 1 - l] = make_float4(oD, oP, __int_as_float(oS), 0.0f);
+        };
         __syncthreads();  // the loader's first ring entries
         for (int p = 0; p < phases; ++p) {
             const int k = p - first;  // this wave's local phase
-            const int s0 = k * T;
-            if (has && s0 + T > 0 && s0 < nsp) {
-                if (k == 0) cur = ring_entries(0);
-                else cur = nxt;
-                nxt = ring_entries(k + 1);  // written before the last barrier
-                if (k == 0) {
-                    if constexpr (FORM == F_TABLE) {
-                        const int nsym = shr1_in_i(0, __float_as_int(lane_in(cur, 0, 2)));
-#pragma unroll
-                        for (int r = 0; r < R; ++r) Snext[r] = tab[rowsym[r] + min(nsym, PAD)];
-                    }
-                }
-                using std::integral_constant;
-                auto phase_steps = [&](auto checked) {
-                    step(s0 + 0, integral_constant<int, 0>{}, DlA, DlB, checked);
-                    step(s0 + 1, integral_constant<int, 1>{}, DlB, DlA, checked);
-                    step(s0 + 2, integral_constant<int, 2>{}, DlA, DlB, checked);
-                    step(s0 + 3, integral_constant<int, 3>{}, DlB, DlA, checked);
-                    if constexpr (T > 4) {
-                        step(s0 + 4, integral_constant<int, 4 % T>{}, DlA, DlB, checked);
-                        step(s0 + 5, integral_constant<int, 5 % T>{}, DlB, DlA, checked);
-                        step(s0 + 6, integral_constant<int, 6 % T>{}, DlA, DlB, checked);
-                        step(s0 + 7, integral_constant<int, 7 % T>{}, DlB, DlA, checked);
-                    }
-                    if constexpr (T > 8) {
-                        step(s0 + 8, integral_constant<int, 8 % T>{}, DlA, DlB, checked);
-                        step(s0 + 9, integral_constant<int, 9 % T>{}, DlB, DlA, checked);
-                        step(s0 + 10, integral_constant<int, 10 % T>{}, DlA, DlB, checked);
-                        step(s0 + 11, integral_constant<int, 11 % T>{}, DlB, DlA, checked);
-                        step(s0 + 12, integral_constant<int, 12 % T>{}, DlA, DlB, checked);
-                        step(s0 + 13, integral_constant<int, 13 % T>{}, DlB, DlA, checked);
-                        step(s0 + 14, integral_constant<int, 14 % T>{}, DlA, DlB, checked);
-                        step(s0 + 15, integral_constant<int, 15 % T>{}, DlB, DlA, checked);
-                    }
-                };
-                if (full && s0 >= WAVE - 1 && s0 + T < lt) phase_steps(integral_constant<bool, false>{});
-                else phase_steps(integral_constant<bool, true>{});
+            if (has && k >= 0 && k < kmax) {
+                const int s0 = k * T - bw;
+                const int obase = (k * T + ooff) & (RING - 1);
+                // every lane on a column inside the problem and before the
+                // last one (whose cells are seeds): no checks
+                if (s0 >= WAVE - 1 && s0 + T < lt) phase_steps(s0, obase, std::false_type{});
+                else phase_steps(s0, obase, std::true_type{});
             }
             __syncthreads();
         }
 
         // ---- this strip's part of the seed search ------------------------------
         if (has) {
+            int bi = bj >= 0 ? row : -1;
             if constexpr (LOC == LOC_LOCAL) {
                 for (int m = 1; m < WAVE; m <<= 1) {
                     const float ob = __shfl_xor(best, m, WAVE);
