@@ -27,7 +27,7 @@ bool long_supported(int gap, int form, int tr) {
 }
 
 size_t long_lds_bytes(int form, int K1) {
-    return size_t(LONG_WAVES + 1) * LONG_RING * 16 + (form == F_TABLE ? size_t(K1) * K1 * 4 : 0);
+    return size_t(LONG_WAVES) * LONG_RING * 16 + (form == F_TABLE ? size_t(K1) * K1 * 4 : 0);
 }
 
 hipError_t launch_long(int loc, int dir, int gap, int form, int tr, bool fold_max, const KArgs &a,

@@ -22,16 +22,19 @@
 // phase's ring entries, at the start of the phase (a read by every lane
 // would move 64 copies of each through the CU's LDS, which the four waves
 // share), so each of its inputs is one DPP move whose fill operand is
-// already in a register; lane 63's outputs pass through a shift register
+// already in a register; lane 63's (D, P) pass through a shift register
 // (lane 0 <- lane 63 by a wave rotate, the rest shifted down a lane) and
-// lanes 0 .. LONG_T - 1 write the phase's entries at its end.  Wave 0's
-// ring holds the strip border (the first band) or the band above's bottom
-// row; the band's last wave writes its bottom row to one more ring, which
-// the loader wave publishes to global memory as two 8-byte words per column
-// tagged with the column (relaxed atomics at device scope: no cache flush).
-// The loader of the band below reads them LONG_K phases before they are
-// due, re-reading (bounded) any whose tag is not there yet.  The tagged
-// buffer is cleared before the launch.
+// lanes 0 .. LONG_T - 1 write the phase's entries at its end.  The loader
+// wave writes every ring entry's symbol a phase before it is read.  Wave
+// 0's ring holds the strip border (the first band) or the band above's
+// bottom row: the band's last wave writes its phase's entries to global memory as
+// two 8-byte words per column tagged with the column (relaxed atomics at
+// device scope: no cache flush), and the loader wave of the band below
+// reads them LONG_K phases before it stages them, re-reading (bounded) any
+// whose tag is not there yet.  A band starts once the band above has
+// published far enough ahead that those reads find their words (a re-read
+// costs the whole workgroup a round trip to L2, and the delays would add up
+// band after band).  The tagged buffer is cleared before the launch.
 //
 // Cells.  The same fold as the general kernel in the reference's push order
 // with strict improvement (pyalign/algorithm/solver.h:1311-1427; linear
@@ -61,9 +64,7 @@ __device__ __forceinline__ int shr1_in_i(int x, int in) {
 __device__ __forceinline__ float push63(float reg, float x) {
     return shr1_in(reg, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x13C, 0xf, 0xf, false)));
 }
-__device__ __forceinline__ int push63_i(int reg, int x) {
-    return shr1_in_i(reg, __builtin_amdgcn_mov_dpp(x, 0x13C, 0xf, 0xf, false));
-}
+
 
 // the step offset of compute wave w behind the phase grid (-w mod LONG_T)
 __device__ __forceinline__ constexpr int long_boff(int w) { return (LONG_T - w % LONG_T) % LONG_T; }
@@ -93,11 +94,11 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
     static_assert(!MX || TR == TR_SCORE, "max folds carry no traceback bits");
 
     extern __shared__ float pa_dyn_lds[];
-    // ring[w] (w < LW): (D, P, symbol bits, -) of the row above compute wave
-    // w's strip; ring[LW]: the band's bottom row, for the loader to publish;
-    // then the similarity table (F_TABLE)
+    // ring[w]: (D, P, symbol bits, -) of the row above compute wave w's
+    // strip at each column (D, P from wave w - 1 or, for wave 0, the
+    // loader; the symbol from the loader); then the similarity table
     float4 *ring = reinterpret_cast<float4 *>(pa_dyn_lds);
-    float *tab = pa_dyn_lds + (LW + 1) * RING * 4;
+    float *tab = pa_dyn_lds + LW * RING * 4;
     __shared__ int s_ticket;
     if constexpr (FORM == F_TABLE) {
         const int n = a.K1 * a.K1;
@@ -128,9 +129,9 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
         const int lt = pr.len_t;
         const int nsp = lt + WAVE - 1;  // steps per strip
         // every wave runs the same number of phases (workgroup barriers): up
-        // to the last wave's last step and one more (the loader publishes
-        // its entries), a multiple of K (the loader's unrolled read-ahead)
-        const int phases = ((LW - 1) * DW + (nsp + T - 1) / T + 2 + K - 1) / K * K;
+        // to the last wave's last step, a multiple of K (the loader's
+        // unrolled read-ahead)
+        const int phases = ((LW - 1) * DW + (nsp + T - 1) / T + 1 + K - 1) / K * K;
         // the band above hands its bottom row down through xb (two 8-byte
         // words per column, tagged with the column + 1)
         uint64_t *xb = reinterpret_cast<uint64_t *>(a.rowbuf + pr.rb_off);
@@ -141,24 +142,6 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
             // the problem) so that no wait is placed at a branch join: the
             // words of phase p are read at phase p - K and waited for only
             // when they are staged.
-            // ---- the band's bottom row: published a phase after the last
-            // wave wrote it (its strip has a successor in the band below)
-            constexpr int wl = LW - 1;
-            const bool pub = band * LW + wl + 1 < nst;
-            uint64_t *xb_out = xb + 2 * int64_t(band) * lt;
-            const float4 *oring = ring + LW * RING;
-            auto publish = [&](int p) {  // the entries the last wave wrote in phase p - 1
-                const int k = p - 1 - wl * DW;
-                const int c = k * T - long_boff(wl) + l - (WAVE - 1);
-                if (pub && k >= 0 && l < T && c >= 0 && c < lt) {
-                    const float4 e = oring[((k * T + long_ooff(wl)) & (RING - 1)) + l];
-                    const uint64_t tag = uint64_t(c + 1) << 32;
-                    __hip_atomic_store(xb_out + 2 * c, tag | __float_as_uint(e.x), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(xb_out + 2 * c + 1, tag | __float_as_uint(e.y), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                }
-            };
             auto run = [&](auto from_band) {
                 constexpr bool XB = decltype(from_band)::value;
                 const uint64_t *xb_in = xb + 2 * int64_t(XB ? band - 1 : 0) * lt;
@@ -207,28 +190,69 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                     }
                 };
                 // wave 0 reads phase k's columns [k T, (k + 1) T) at the start
-                // of phase k - 1 (phases 0 and 1 at the start of phase 0):
-                // lane l < T stages column due(p) = (p + 2) T + l at phase p
+                // of phase k (the table form also column (k + 1) T): lane
+                // l < T stages column due(p) = (p + DUE) T + l at phase p
+                constexpr int DUE = FORM == F_TABLE ? 2 : 1;
                 const bool mine = l < T;
-                auto due = [&](int p) { return (p + 2) * T + (l < T ? l : T - 1); };
-                {   // phases 0 and 1: columns [0, 2 T)
-                    const bool m0 = l < 2 * T;
+                auto due = [&](int p) { return (p + DUE) * T + (l < T ? l : T - 1); };
+                if constexpr (XB) {
+                    // start once the band above has published the column
+                    // this band's read-ahead reaches a few phases in
+                    const int c0 = min(lt - 1, (DUE + K + 3) * T - 1);
+                    const uint32_t tag = uint32_t(c0 + 1);
+                    uint32_t spins = 0;
+                    while (uint32_t(__hip_atomic_load(xb_in + 2 * c0 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >>
+                                    32) != tag) {
+                        if (++spins > LONG_SPIN_LIMIT ||
+                            ((spins & 63u) == 0 &&
+                             __hip_atomic_load(a.band_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                            __hip_atomic_store(a.band_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                }
+                {   // the first phases: columns [0, DUE T)
+                    const bool m0 = l < DUE * T;
                     uint64_t d, p;
                     uint32_t sy;
                     issue(l, d, p, sy);
                     stage(l, m0, d, p, sy);
                 }
+                // the symbols of compute wave sw = 1 + l / T (lanes below
+                // (LW - 1) T): its local phase k's at global phase
+                // P_sw + k - DUE, column k T - b_sw + l mod T into slot
+                // (k T + l mod T) mod RING
+                const int sw = 1 + l / T, st_sw = band * LW + sw;
+                const bool sym_lane = sw < LW && st_sw < nst;
+                const int sk0 = DUE - sw * DW;  // local phase = global phase + sk0
+                auto sym_col = [&](int p) { return (p + sk0) * T - long_boff(sw) + (l & (T - 1)); };
+                auto sym_issue = [&](int p, uint32_t &sy) {
+                    const int cc = min(max(sym_col(p), 0), lt - 1);
+                    if constexpr (FORM == F_TABLE) sy = ((const uint8_t *)a.codes_t)[pr.off_t + cc];
+                    else sy = ((const uint32_t *)a.codes_t)[pr.off_t + cc];
+                };
+                auto sym_stage = [&](int p, uint32_t sy) {
+                    const int k = p + sk0;
+                    if (sym_lane && k >= 0)
+                        reinterpret_cast<float *>(ring + sw * RING + ((k * T + (l & (T - 1))) & (RING - 1)))[2] =
+                            __uint_as_float(sy);
+                };
                 uint64_t hD[K], hP[K];
-                uint32_t hs[K];
+                uint32_t hs[K], hy[K];
 #pragma unroll
-                for (int u = 0; u < K; ++u) issue(due(u), hD[u], hP[u], hs[u]);
+                for (int u = 0; u < K; ++u) {
+                    issue(due(u), hD[u], hP[u], hs[u]);
+                    sym_issue(u, hy[u]);
+                }
                 __syncthreads();  // the compute waves' first ring entries
                 for (int p0 = 0; p0 < phases; p0 += K) {
 #pragma unroll
                     for (int u = 0; u < K; ++u) {
                         stage(due(p0 + u), mine, hD[u], hP[u], hs[u]);
-                        publish(p0 + u);
+                        sym_stage(p0 + u, hy[u]);
                         issue(due(p0 + u + K), hD[u], hP[u], hs[u]);
+                        sym_issue(p0 + u + K, hy[u]);
                         __syncthreads();
                     }
                 }
@@ -251,7 +275,10 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
         const int first = wave * DW;  // this wave's local phase 0
         const int kmax = (nsp + bw + T - 1) / T;  // its phases: steps -b_w .. nsp - 1
         const float4 *in_ring = ring + wave * RING;
-        float4 *out_ring = ring + (wave + 1) * RING;
+        float4 *out_ring = ring + (wave + 1) * RING;  // (wave + 1 < LW)
+        // the band's last wave hands its rows to the band below
+        const bool to_band = wave == LW - 1 && st + 1 < nst;
+        uint64_t *xb_out = xb + 2 * int64_t(band) * lt;
         const int ooff = long_ooff(wave);
         const float go_s = a.go_s, ge_s = a.ge_s, go_t = a.go_t, ge_t = a.ge_t;
         [[maybe_unused]] const int PAD = a.K1 - 1;
@@ -291,9 +318,8 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
 
         // one step: column j = s - l; `e` is lane 0's ring entry (column s),
         // `en` the next step's
-        // lane 63's last T outputs (D, P, symbol), newest in lane 0
+        // lane 63's last T outputs (D, P), newest in lane 0
         float oD = W, oP = W;
-        int oS = 0;
         auto body = [&](int s, const float4 &e, const float4 &en, auto checked) {
             constexpr bool CHK = decltype(checked)::value;
             const float inD = shr1_in(botD, e.x);
@@ -406,7 +432,6 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
             // lane 63 hands its bottom row (column s - 63) down
             oD = push63(oD, botD);
             oP = push63(oP, botP);
-            oS = push63_i(oS, msym);
         };
 
         // a phase's ring entries are read by lane 0 at its start (written in
@@ -425,8 +450,21 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
             PA_LSTEP(8); PA_LSTEP(9); PA_LSTEP(10); PA_LSTEP(11); PA_LSTEP(12); PA_LSTEP(13); PA_LSTEP(14);
             PA_LSTEP(15);
 #undef PA_LSTEP
-            // lane k holds step T - 1 - k's output
-            if (l < T) out_ring[obase + T - 1 - l] = make_float4(oD, oP, __int_as_float(oS), 0.0f);
+            // lane k holds step T - 1 - k's output (column s0 + T - 1 - k - 63)
+            if (l < T) {
+                if (wave + 1 < LW) {
+                    *reinterpret_cast<float2 *>(out_ring + obase + T - 1 - l) = make_float2(oD, oP);
+                } else if (to_band) {
+                    const int c = s0 + T - 1 - l - (WAVE - 1);
+                    if (c >= 0 && c < lt) {
+                        const uint64_t tag = uint64_t(c + 1) << 32;
+                        __hip_atomic_store(xb_out + 2 * c, tag | __float_as_uint(oD), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(xb_out + 2 * c + 1, tag | __float_as_uint(oP), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+            }
         };
         __syncthreads();  // the loader's first ring entries
         for (int p = 0; p < phases; ++p) {
