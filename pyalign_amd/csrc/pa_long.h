@@ -320,16 +320,23 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
         // `en` the next step's
         // lane 63's last T outputs (D, P), newest in lane 0
         float oD = W, oP = W;
+        // checks a phase needs (bits): LO, some lanes before column 0 (the
+        // strip's first phases); HI, some lanes at or past the last column
+        // (its last phases); NEG, steps before the strip's first step (its
+        // first phase)
+        constexpr int LO = 1, HI = 2, NEG = 4;
         auto body = [&](int s, const float4 &e, const float4 &en, auto checked) {
-            constexpr bool CHK = decltype(checked)::value;
+            constexpr int CHK = decltype(checked)::value;
             const float inD = shr1_in(botD, e.x);
             const float inP = shr1_in(botP, e.y);
             const int tsym = shr1_in_i(msym, __float_as_int(e.z));
             msym = tsym;
             float S;
             if constexpr (FORM == F_TABLE) {
-                if constexpr (CHK) S = tab[rowsym + int(min(uint32_t(tsym), uint32_t(PAD)))];
-                else S = Snext;
+                if constexpr ((CHK & LO) != 0) {
+                    if (s == 0) Snext = tab[rowsym + int(min(uint32_t(tsym), uint32_t(PAD)))];  // (no step before)
+                }
+                S = Snext;
                 // the next step's symbol is the previous lane's current one
                 const int nsym = shr1_in_i(tsym, __float_as_int(en.z));
                 Snext = tab[rowsym + int(min(uint32_t(nsym), uint32_t(PAD)))];
@@ -339,7 +346,10 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
             const float Dd = prevD;
             prevD = inD;
             const int j = s - l;
-            if (!CHK || (j >= 0 && j < lt)) {
+            // checked steps: every lane computes a cell, those whose column
+            // lies outside the problem keep their state (selects, no branch)
+            const bool act = ((CHK & LO) == 0 || l <= s) && ((CHK & HI) == 0 || l > s - lt);
+            {
                 float D, P = 0.0f, Q = 0.0f;
                 [[maybe_unused]] uint32_t bits = 0;
                 const float dg = Dd + S;
@@ -389,11 +399,12 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                 }
                 if constexpr (LOC == LOC_LOCAL) {
                     if constexpr (TR == TR_SCORE) {
-                        best = MX ? opt(best, D) : (better<DIR>(D, best) ? D : best);  // padded rows stay below
+                        const float nb = MX ? opt(best, D) : (better<DIR>(D, best) ? D : best);  // padded rows stay below
+                        best = act ? nb : best;
                     } else {
                         // Optima over i desc, j desc (solver.h:2050-2056): in
                         // one row a later column wins a tie
-                        if (rv && (DIR == DIR_MAX ? D >= best : D <= best)) {
+                        if (act && rv && (DIR == DIR_MAX ? D >= best : D <= best)) {
                             best = D;
                             bj = j;
                         }
@@ -402,33 +413,37 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                     // every lane tracks its row; the last row's lane is kept
                     // (last row, j ascending: solver.h:2285-2287)
                     if constexpr (TR == TR_SCORE && MX) {
-                        rbest = opt(rbest, D);
-                    } else if (better<DIR>(D, rbest)) {
+                        rbest = act ? opt(rbest, D) : rbest;
+                    } else if (act && better<DIR>(D, rbest)) {
                         rbest = D;
                         rj = j;
                     }
-                    if constexpr (CHK) {
+                    if constexpr ((CHK & HI) != 0) {
                         if (rv && j == lt - 1) {  // last column (solver.h:2289-2291)
                             cbest = D;
                             ci = row;
                         }
                     }
                 } else {
-                    if constexpr (CHK) {
+                    if constexpr ((CHK & HI) != 0) {
                         if (j == lt - 1) gscore = D;
                     }
                 }
-                if constexpr (TR == TR_ONE) tbp[int64_t(s) * WAVE] = bits;
-                Dl = D;
-                Ql = Q;
-                botD = D;
-                botP = P;
+                // (the words of lanes outside the problem lie in the strip's
+                // area and are never read; steps past the strip have none)
+                if constexpr (TR == TR_ONE) {
+                    if ((CHK & HI) == 0 || s < nsp) tbp[int64_t(s) * WAVE] = bits;
+                }
+                Dl = act ? D : Dl;
+                Ql = act ? Q : Ql;
+                botD = act ? D : botD;
+                botP = act ? P : botP;
             }
         };
 
         auto step = [&](int s, const float4 &e, const float4 &en, auto checked) {
-            constexpr bool CHK = decltype(checked)::value;
-            if (!CHK || s >= 0) body(s, e, en, checked);  // (steps before the strip's first do nothing)
+            constexpr int CHK = decltype(checked)::value;
+            if ((CHK & NEG) == 0 || s >= 0) body(s, e, en, checked);  // (steps before the strip's first do nothing)
             // lane 63 hands its bottom row (column s - 63) down
             oD = push63(oD, botD);
             oP = push63(oP, botP);
@@ -474,8 +489,17 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                 const int obase = (k * T + ooff) & (RING - 1);
                 // every lane on a column inside the problem and before the
                 // last one (whose cells are seeds): no checks
-                if (s0 >= WAVE - 1 && s0 + T < lt) phase_steps(s0, obase, std::false_type{});
-                else phase_steps(s0, obase, std::true_type{});
+                using std::integral_constant;
+                const bool lo = s0 < WAVE - 1, hi = s0 + T >= lt;
+                if (!lo && !hi) phase_steps(s0, obase, integral_constant<int, 0>{});
+                else if (!hi) {
+                    if (s0 >= 0) phase_steps(s0, obase, integral_constant<int, LO>{});
+                    else phase_steps(s0, obase, integral_constant<int, LO | NEG>{});
+                } else if (!lo) {
+                    phase_steps(s0, obase, integral_constant<int, HI>{});
+                } else {
+                    phase_steps(s0, obase, integral_constant<int, LO | HI | NEG>{});
+                }
             }
             __syncthreads();
         }
