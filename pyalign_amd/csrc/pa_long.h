@@ -198,7 +198,7 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                 if constexpr (XB) {
                     // start once the band above has published the column
                     // this band's read-ahead reaches a few phases in
-                    const int c0 = min(lt - 1, (DUE + K + 3) * T - 1);
+                    const int c0 = min(lt - 1, (DUE + K + 1) * T - 1);
                     const uint32_t tag = uint32_t(c0 + 1);
                     uint32_t spins = 0;
                     while (uint32_t(__hip_atomic_load(xb_in + 2 * c0 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >>
