@@ -116,7 +116,7 @@ bool supported_R(int form, int tr, bool fastmax, int R) {
 #define PA_WALK_ST_MUL 1
 #endif
 #ifndef PA_WALK_LOADS
-#define PA_WALK_LOADS 8
+#define PA_WALK_LOADS 32
 #endif
 struct WalkShape {
     int waves;       // wavefronts (problems) per block
@@ -243,6 +243,9 @@ struct TbWalker {
             const int cnt = (s1 - s0) * kw;
             const uint32_t *base = tb + tb_off + (int64_t(t_st) * nsteps + s0) * L * wpc + int64_t(k0) * wpc;
             const int sstride = L * wpc;
+            // w / kw as w * (1 / kw) in float (w < 2^24: exact operands),
+            // corrected by one step where the rounded product falls short
+            const float rkw = 1.0f / float(kw);
             __builtin_amdgcn_wave_barrier();  // the old tile's reads are done
             // PA_WALK_LOADS loads in flight per lane before their LDS stores
             constexpr int U = PA_WALK_LOADS;
@@ -251,7 +254,9 @@ struct TbWalker {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int w = w0 + u * WAVE + lane;
-                    const int si = w / kw;
+                    int si = int(float(w) * rkw);
+                    si += (si + 1) * kw <= w ? 1 : 0;
+                    si -= si * kw > w ? 1 : 0;
                     v[u] = w < cnt ? base[int64_t(si) * sstride + (w - si * kw)] : 0u;
                 }
 #pragma unroll
