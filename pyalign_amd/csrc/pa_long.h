@@ -325,7 +325,10 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
         // (its last phases); NEG, steps before the strip's first step (its
         // first phase)
         constexpr int LO = 1, HI = 2, NEG = 4;
-        auto body = [&](int s, const float4 &e, const float4 &en, auto checked) {
+        // tbq: this lane's traceback word of the phase's first step (step
+        // s0 + t's is tbq[t * WAVE]: constant offsets within a phase)
+        auto body = [&](int s, uint32_t *tbq, auto tc, const float4 &e, const float4 &en, auto checked) {
+            [[maybe_unused]] constexpr int t = decltype(tc)::value;
             constexpr int CHK = decltype(checked)::value;
             const float inD = shr1_in(botD, e.x);
             const float inP = shr1_in(botP, e.y);
@@ -432,7 +435,7 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                 // (the words of lanes outside the problem lie in the strip's
                 // area and are never read; steps past the strip have none)
                 if constexpr (TR == TR_ONE) {
-                    if ((CHK & HI) == 0 || s < nsp) tbp[int64_t(s) * WAVE] = bits;
+                    if ((CHK & HI) == 0 || s < nsp) tbq[t * WAVE] = bits;
                 }
                 Dl = act ? D : Dl;
                 Ql = act ? Q : Ql;
@@ -441,9 +444,9 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
             }
         };
 
-        auto step = [&](int s, const float4 &e, const float4 &en, auto checked) {
+        auto step = [&](int s, uint32_t *tbq, auto tc, const float4 &e, const float4 &en, auto checked) {
             constexpr int CHK = decltype(checked)::value;
-            if ((CHK & NEG) == 0 || s >= 0) body(s, e, en, checked);  // (steps before the strip's first do nothing)
+            if ((CHK & NEG) == 0 || s >= 0) body(s, tbq, tc, e, en, checked);  // (steps before the strip's first do nothing)
             // lane 63 hands its bottom row (column s - 63) down
             oD = push63(oD, botD);
             oP = push63(oP, botP);
@@ -452,6 +455,8 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
         // a phase's ring entries are read by lane 0 at its start (written in
         // an earlier phase); the first step waits for the first of them
         auto phase_steps = [&](int s0, int obase, auto checked) {
+            using std::integral_constant;
+            uint32_t *const tbq = TR == TR_ONE ? tbp + int64_t(s0) * WAVE : nullptr;
             float4 cur[T + 1];
             if (l == 0) {
                 const float4 *src = in_ring + ((s0 + bw) & (RING - 1));
@@ -459,8 +464,9 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                 for (int t = 0; t < T; ++t) cur[t] = src[t];
                 if constexpr (FORM == F_TABLE) cur[T] = in_ring[(s0 + bw + T) & (RING - 1)];
             }
-#define PA_LSTEP(t) \
-    if constexpr ((t) < T) step(s0 + (t), cur[(t) % T], cur[(t) + 1 <= T ? (t) + 1 : T], checked)
+#define PA_LSTEP(t)                                                                                       \
+    if constexpr ((t) < T) step(s0 + (t), tbq, integral_constant<int, (t) % T>{}, cur[(t) % T],              \
+                                cur[(t) + 1 <= T ? (t) + 1 : T], checked)
             PA_LSTEP(0); PA_LSTEP(1); PA_LSTEP(2); PA_LSTEP(3); PA_LSTEP(4); PA_LSTEP(5); PA_LSTEP(6); PA_LSTEP(7);
             PA_LSTEP(8); PA_LSTEP(9); PA_LSTEP(10); PA_LSTEP(11); PA_LSTEP(12); PA_LSTEP(13); PA_LSTEP(14);
             PA_LSTEP(15);
