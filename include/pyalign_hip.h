@@ -242,6 +242,11 @@ int pa_solver_set_graphs(pa_solver *solver, int32_t enable);
  * pipeline), and launch sequences are not captured as graphs.  A tape's
  * forward and backward must use the same stream.  No reference counterpart. */
 int pa_solver_set_stream(pa_solver *solver, void *stream, int32_t external);
+
+/* The stream the solver's device work goes to (its own, or the caller's set
+ * by pa_solver_set_stream), so that a caller can order events or work of its
+ * own with the solver's.  No reference counterpart. */
+int pa_solver_stream(const pa_solver *solver, void **stream);
 /* Checkpointed differentiation: in pa_forward_ragged, a global problem on a
  * device-resident dense or series payload whose longer side exceeds min_len
  * (default 2048) keeps only the bottom row of each strip of 512 rows, not

@@ -68,6 +68,7 @@ SIGNATURES = {
     "pa_solver_set_graphs": (ctypes.c_int, [_P, _I32]),
     "pa_solver_set_checkpoint": (ctypes.c_int, [_P, _I64]),
     "pa_solver_set_stream": (ctypes.c_int, [_P, _P, _I32]),
+    "pa_solver_stream": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_void_p)]),
     "pa_solver_set_gap_costs": (ctypes.c_int, [_P, _P, _I64, _P, _I64]),
     "pa_solver_set_gap_params": (ctypes.c_int, [_P, _P, _P]),
     "pa_solver_last_bad_symbol": (ctypes.c_int, [_P, _P, _P]),

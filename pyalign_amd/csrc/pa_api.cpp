@@ -2152,6 +2152,13 @@ int pa_solver_sync(pa_solver *s) {
     });
 }
 
+int pa_solver_stream(const pa_solver *s, void **stream) {
+    return guard([&] {
+        if (!s || !stream) fail(PA_EINVAL, "null argument");
+        *stream = reinterpret_cast<void *>(s->stream);
+    });
+}
+
 int pa_solver_last_kernel_ms(const pa_solver *s, double *ms, int32_t *launches, double *cells) {
     return guard([&] {
         if (!s) fail(PA_EINVAL, "null solver");
