@@ -307,11 +307,13 @@ __device__ __forceinline__ int walk_step(TbWalker &w, int lane) {
 // words per lane and load.  Returns the path's length; cells are stored as
 // pa_walk_kernel stores them.
 template <int GAP>
-__device__ int walk_rows64(const Prob &pr, const uint32_t *tb, uint32_t *lds, int tile_words, int u, int v,
+__device__ __forceinline__ int walk_rows64(const Prob &pr, const uint32_t *tb, uint32_t *lds, int tile_words, int u, int v,
                            int32_t *P, int lane) {
     constexpr uint32_t MASK = GAP == GAP_AFF ? 15u : 3u;
     const int nsteps = pr.len_t + WAVE - 1;
-    const int ST = tile_words / WAVE;  // tile rows (steps)
+    // tile rows (steps): a diagonal path crosses a strip's 64 lanes in 128
+    // steps; the whole tile is one round of loads (32 per lane)
+    const int ST = tile_words / WAVE < 128 ? tile_words / WAVE : 128;
     int i = u, j = v;
     int st = i / WAVE, k = i - st * WAVE, s = j + k;
     int t_st = -1, s0 = 0, idx = 0;
@@ -325,21 +327,18 @@ __device__ int walk_rows64(const Prob &pr, const uint32_t *tb, uint32_t *lds, in
         const int cnt = (s1 - s0) * WAVE;  // words, a multiple of 64
         __builtin_amdgcn_wave_barrier();   // the old tile's reads are done
         if (vec) {
+            // loads straight into LDS (16 bytes a lane, lane l at the
+            // instruction's base + 16 l): the whole tile in one round, no
+            // registers held
             const uint4 *b4 = reinterpret_cast<const uint4 *>(base);
-            constexpr int U = 16;
-            for (int c0 = 0; c0 < cnt / 4; c0 += U * WAVE) {
-                uint4 x[U];
-#pragma unroll
-                for (int q = 0; q < U; ++q) {
-                    const int c = c0 + q * WAVE + lane;
-                    x[q] = c < cnt / 4 ? b4[c] : make_uint4(0u, 0u, 0u, 0u);
-                }
-#pragma unroll
-                for (int q = 0; q < U; ++q) {
-                    const int c = c0 + q * WAVE + lane;
-                    if (c < cnt / 4) reinterpret_cast<uint4 *>(lds)[c] = x[q];
-                }
+            const int n4 = cnt / 4;
+            for (int q = 0; q * WAVE < n4; ++q) {
+                if (q * WAVE + lane < n4)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(b4 + q * WAVE + lane),
+                        (__attribute__((address_space(3))) void *)(lds + q * WAVE * 4), 16, 0, 0);
             }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         } else {
             constexpr int U = 32;
             for (int c0 = 0; c0 < cnt; c0 += U * WAVE) {
@@ -360,14 +359,37 @@ __device__ int walk_rows64(const Prob &pr, const uint32_t *tb, uint32_t *lds, in
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
+    // A window of the cells the path can reach in its next few moves, one
+    // word per lane: lane 6 a + b holds the cell a left or up moves and b
+    // up moves on (tile index idx - 64 a - b, a < 10, b < 6), read once;
+    // a move then takes its code from a register of that lane (readlane)
+    // instead of an LDS round trip.  (wa, wb) is the current cell's place in
+    // the window; leaving it, or a new tile, reads a new window.
+    constexpr int WB = 6, WA = 10;
+    uint32_t win = 0;
+    int wa = WA, wb = 0;
+    auto refresh = [&]() {
+        wa = wb = 0;
+        const int q = lane < WA * WB ? lane : 0;
+        const int a = q / WB, b = q - a * WB;
+        const int x = idx - WAVE * a - b;
+        win = lds[x > 0 ? x : 0];
+    };
     auto code = [&]() -> uint32_t {
-        if (st != t_st || s < s0) load();
-        return __builtin_amdgcn_readfirstlane(lds[idx]) & MASK;
+        if (st != t_st || s < s0) {
+            load();
+            refresh();
+        } else if (wa >= WA || wb >= WB) {
+            refresh();
+        }
+        return uint32_t(__builtin_amdgcn_readlane(int(win), wa * WB + wb)) & MASK;
     };
     auto up = [&]() {
         --i;
         --s;
         idx -= WAVE + 1;
+        ++wa;
+        ++wb;
         if (--k < 0) {  // into the strip above (its tile is loaded on the next read)
             k = WAVE - 1;
             s += WAVE;
@@ -378,6 +400,7 @@ __device__ int walk_rows64(const Prob &pr, const uint32_t *tb, uint32_t *lds, in
         --j;
         --s;
         idx -= WAVE;
+        ++wa;
     };
     int pi = u, pj = v;  // cell 0 sits in lane 0
     int cnt = 1;
