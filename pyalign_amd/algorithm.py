@@ -540,12 +540,14 @@ class Solver:
         return self._binary(a, b, eq, ne, length, "solution_iterator")
 
     # ---- whole-bag entry (no reference counterpart: one call per bag) --
-    def solve_ragged(self, kind, form, len_s, len_t, schedule=_lib.PA_SCHED_AUTO, **payload):
+    def solve_ragged(self, kind, form, len_s, len_t, schedule=_lib.PA_SCHED_AUTO, *, payload_owned=False, **payload):
         """Solve n problems of individual lengths in one call.
 
         form: "indexed" (codes_s/codes_t uint8, off_s/off_t, table KxK),
         "binary" (uint32 codes, eq, ne), "dense" (dense float32 flat,
         dense_off) or "series" (float32 rows x dim).  kind as in _wrap.
+        payload_owned: the payload's arrays belong to this call (nobody
+        changes them later), so Solutions keep them without a copy.
         """
         rg, keep = make_ragged(form, len_s, len_t, **payload)
         h = ctypes.c_void_p()
@@ -553,7 +555,7 @@ class Solver:
         del keep
         one = None
         if kind.startswith("solution") and not payload.get("on_device", False):
-            one = _ragged_one(form, len_s, len_t, payload)
+            one = _ragged_one(form, len_s, len_t, payload, copy=not payload_owned)
         return self._wrap(h, len_s, len_t, kind, one)
 
     def set_checkpoint(self, min_len):
@@ -624,13 +626,15 @@ class _Deferred:
         return self._m()
 
 
-def _ragged_one(form, len_s, len_t, p):
+def _ragged_one(form, len_s, len_t, p, copy=True):
     """Problem k's data out of a ragged payload, as a one-problem payload
-    (the payload's arrays are copied once here, so later changes by the
-    caller do not reach solutions built from them)."""
+    (a caller's arrays are copied once here, so later changes by the caller
+    do not reach solutions built from them; arrays the solve built itself
+    are kept as they are)."""
     ls = np.array(len_s, dtype=np.int64)
     lt = np.array(len_t, dtype=np.int64)
-    p = {key: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for key, v in p.items()}
+    if copy:
+        p = {key: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for key, v in p.items()}
 
     def one(k):
         a, b = int(ls[k]), int(lt[k])

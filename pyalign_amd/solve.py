@@ -12,6 +12,7 @@ SolverCache does.
 
 import contextlib
 import gc
+import threading
 import itertools
 import time
 import typing
@@ -439,15 +440,29 @@ def _collector_paused():
     traverse every live object of the process (4,096 Solutions of 100 x 100
     global alignments: 13 ms per solve paused, 21 ms median and up to 110 ms
     beside 2 M live objects without).  Reference counting still frees
-    acyclic garbage at once; cycles wait for the next collection."""
-    if not gc.isenabled():
-        yield
-        return
-    gc.disable()
+    acyclic garbage at once; cycles wait for the next collection.
+
+    The collector's switch is process-wide: builds in several threads share
+    one pause (a count under a lock), which ends with the last of them, and
+    a collector that was off at the first of them stays off."""
+    global _pause_count, _pause_restore
+    with _pause_lock:
+        if _pause_count == 0:
+            _pause_restore = gc.isenabled()
+            gc.disable()
+        _pause_count += 1
     try:
         yield
     finally:
-        gc.enable()
+        with _pause_lock:
+            _pause_count -= 1
+            if _pause_count == 0 and _pause_restore:
+                gc.enable()
+
+
+_pause_lock = threading.Lock()
+_pause_count = 0
+_pause_restore = False
 
 
 def _joined_text(problems):
@@ -595,7 +610,7 @@ class Solver:
                     raise ValueError("out of bounds index in b")
                 code_t = np.uint16 if max(table.shape) > 128 else np.uint8
                 payload = dict(codes_s=cs.astype(code_t), codes_t=ct.astype(code_t), off_s=os_, off_t=ot,
-                               table=np.asarray(table, dtype=np.float32))
+                               table=np.array(table, dtype=np.float32))  # (a copy: the caller's table may change)
                 form = "indexed"
             elif bag.form == Form.BINARY_MATRIX_FORM:
                 eq, ne = _shared_eq_ne(bag)
@@ -618,16 +633,18 @@ class Solver:
                     p.build_matrix(dense[o:o + a * b].reshape(a, b))
                 payload = dict(dense=dense, dense_off=dense_off)
                 form = "dense"
-        with self._timings.measure("solve"), _collector_paused():
+        with self._timings.measure("solve"):
             try:
-                r = solver.solve_ragged(kind, form, ls, lt, **payload)
+                # every payload array above was built for this call
+                r = solver.solve_ragged(kind, form, ls, lt, payload_owned=True, **payload)
             except ValueError as e:
                 if text is None or "not in alphabet" not in str(e):
                     raise
                 raise _bad_symbol_error(solver, text, payload["off_s"], payload["off_t"]) from None
             if kind == "score":  # the construct is the identity
                 return list(r)
-            return [construct(p, solver, x) for p, x in zip(problems, r)]
+            with _collector_paused():
+                return [construct(p, solver, x) for p, x in zip(problems, r)]
 
     def _solve_problem(self, problem):
         return self._solve_bag(ProblemBag([problem]))[0]

@@ -200,3 +200,36 @@ def test_text_payload_layout():
     assert payload["off_s"].tolist() == [0, 3, 3] and payload["off_t"].tolist() == [0, 2, 6]
     probs = [pf.new_problem(list("ACD"), "EF")]
     assert S._text_payload(probs, ls[:1], lt[:1], True) == (None, None)
+
+
+def test_collector_pause_is_shared_by_concurrent_builds():
+    """The cyclic collector stays paused until the last concurrent result
+    build ends, and one that was off before stays off."""
+    import gc
+    import threading
+    from pyalign_amd import solve as S
+
+    assert gc.isenabled()
+    inside, release = threading.Event(), threading.Event()
+
+    def other():
+        with S._collector_paused():
+            inside.set()
+            release.wait(5)
+
+    th = threading.Thread(target=other)
+    th.start()
+    assert inside.wait(5)
+    with S._collector_paused():
+        assert not gc.isenabled()
+    assert not gc.isenabled()  # the other thread's build is still running
+    release.set()
+    th.join(5)
+    assert gc.isenabled()
+    gc.disable()
+    try:
+        with S._collector_paused():
+            assert not gc.isenabled()
+        assert not gc.isenabled()
+    finally:
+        gc.enable()
