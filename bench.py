@@ -336,7 +336,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the long-pair and fwd+bwd measurements")
-    ap.add_argument("--only", choices=["fwd_bwd", "dtw_fwd_bwd", "long_pair"], default=None,
+    ap.add_argument("--only", choices=["fwd_bwd", "dtw_fwd_bwd", "long_pair", "global_solution"], default=None,
                     help="only time one side workload (differentiable: 5 steps / 1 step; long pair: 5 solves) "
                          "and print its JSON (profiling)")
     args = ap.parse_args()
@@ -360,7 +360,8 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
         return
     if args.only:
-        only = {"fwd_bwd": lambda: c4_bench(5), "dtw_fwd_bwd": lambda: c5_bench(1), "long_pair": lambda: c3_bench(5)}
+        only = {"fwd_bwd": lambda: c4_bench(5), "dtw_fwd_bwd": lambda: c5_bench(1), "long_pair": lambda: c3_bench(5),
+                "global_solution": lambda: c1_bench(10)}
         print(json.dumps(only[args.only]()))
         return
     table, ls, lt, codes_s, codes_t, off_s, off_t = c2_inputs()

@@ -64,6 +64,19 @@ def test_indexed_text_equals_host_encoding(name, codomain, monkeypatch):
         else:
             assert np.float32(h.score).tobytes() == np.float32(d.score).tobytes(), (a, b)
             assert np.array_equal(np.asarray(h.path), np.asarray(d.path)), (a, b)
+    # and every tenth problem against the oracle
+    lims = (rounded(max(len(a) for a, _ in seqs)), rounded(max(len(b) for _, b in seqs)))
+    goal = O.GOAL_SCORE if codomain is pa.Score else O.GOAL_ONE
+    for k in range(0, len(seqs), 10):
+        a, b = seqs[k]
+        if not a or not b:  # (empty problems: tests/test_gpu_parity.py)
+            continue
+        ref = O.OracleResult(O.make_config(locality=O.LOCAL, gap=("affine", 3, 1), goal=goal,
+                                           lim_s=lims[0], lim_t=lims[1]), dense(pf.new_problem(list(a), list(b))))
+        got = dev[k] if codomain is pa.Score else dev[k].score
+        assert np.float32(got).tobytes() == ref.score.tobytes(), (k, a, b)
+        if codomain is pa.Solution:
+            assert np.array_equal(np.asarray(dev[k].path, dtype=np.int32), ref.path()), (k, a, b)
 
 
 def test_indexed_text_against_oracle_and_matrices(monkeypatch):
@@ -104,6 +117,17 @@ def test_binary_text_equals_host_encoding(text, monkeypatch):
     for (a, b), h, d in zip(seqs, host, dev):
         assert np.float32(h.score).tobytes() == np.float32(d.score).tobytes(), (a, b)
         assert np.array_equal(np.asarray(h.path), np.asarray(d.path)), (a, b)
+    # and every tenth problem against the oracle
+    lims = (rounded(max(len(a) for a, _ in seqs)), rounded(max(len(b) for _, b in seqs)))
+    for k in range(0, len(seqs), 10):
+        a, b = seqs[k]
+        if not a or not b:
+            continue
+        S = np.where(np.array([ord(x) for x in a])[:, None] == np.array([ord(y) for y in b])[None, :], 2, -1)
+        ref = O.OracleResult(O.make_config(locality=O.SEMIGLOBAL, gap=("linear", 1), goal=O.GOAL_ONE,
+                                           lim_s=lims[0], lim_t=lims[1]), S.reshape(len(a), len(b)).astype(np.float32))
+        assert np.float32(dev[k].score).tobytes() == ref.score.tobytes(), (k, a, b)
+        assert np.array_equal(np.asarray(dev[k].path, dtype=np.int32), ref.path()), (k, a, b)
 
 
 def test_symbol_outside_alphabet_errors():
