@@ -1343,12 +1343,13 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     const int n_band = int(plan.band_probs.size());
     LongArgs la{};
     int long_grid = 0;
-    // the long kernel folds score-only cells with max / min where that keeps
-    // the strict fold's bits: similarities finite (or the pad's worst value)
-    // and never -0, and, for maximising global problems, gap costs above zero
-    // (borders below -0), so that no candidate is -0 or NaN
+    // the long kernel folds cells with max / min (and takes traceback codes
+    // from equality tests) where that keeps the strict fold's bits:
+    // similarities finite (or the pad's worst value) and never -0, and, for
+    // maximising global problems, gap costs above zero (borders below -0), so
+    // that no candidate is -0 or NaN
     bool fold_max = false;
-    if (plan.n_long > 0 && job.tr == TR_SCORE) {
+    if (plan.n_long > 0) {
         auto exact = [](float x) { return std::isfinite(x) && !(x == 0.0f && std::signbit(x)); };
         bool good = true;
         if (job.form == F_BINARY) {

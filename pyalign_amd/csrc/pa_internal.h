@@ -182,8 +182,9 @@ constexpr int FAST_BLOCK = PA_FAST_BLOCK;
 hipError_t launch_solve(const LaunchSpec &spec, const KArgs &args, hipStream_t stream);
 // Long-problem mode (forms F_TABLE / F_BINARY, score or one traceback,
 // linear or affine gaps); `grid` workgroups take the bands in order.
-// fold_max: max / min folds instead of strict compare-and-select (score
-// only; the caller has checked that they return the same bits).
+// fold_max: max / min folds instead of strict compare-and-select, traceback
+// codes from equality tests (the caller has checked that they return the
+// same bits).
 hipError_t launch_long(int loc, int dir, int gap, int form, int tr, bool fold_max, const KArgs &args,
                        const LongArgs &la, int grid, hipStream_t stream);
 size_t long_lds_bytes(int form, int K1);

@@ -34,18 +34,17 @@ hipError_t launch_long(int loc, int dir, int gap, int form, int tr, bool fold_ma
                        const LongArgs &la, int grid, hipStream_t st) {
     if (!long_supported(gap, form, tr) || la.n <= 0 || grid <= 0) return hipErrorInvalidValue;
     const size_t lds = long_lds_bytes(form, a.K1);
-    const bool fm = fold_max && tr == TR_SCORE;
     const bool mx = dir == DIR_MAX;
     switch (loc) {
         case LOC_LOCAL:
-            return mx ? dispatch_long_local_max(gap, form, tr, fm, a, la, grid, lds, st)
-                      : dispatch_long_local_min(gap, form, tr, fm, a, la, grid, lds, st);
+            return mx ? dispatch_long_local_max(gap, form, tr, fold_max, a, la, grid, lds, st)
+                      : dispatch_long_local_min(gap, form, tr, fold_max, a, la, grid, lds, st);
         case LOC_GLOBAL:
-            return mx ? dispatch_long_global_max(gap, form, tr, fm, a, la, grid, lds, st)
-                      : dispatch_long_global_min(gap, form, tr, fm, a, la, grid, lds, st);
+            return mx ? dispatch_long_global_max(gap, form, tr, fold_max, a, la, grid, lds, st)
+                      : dispatch_long_global_min(gap, form, tr, fold_max, a, la, grid, lds, st);
         case LOC_SEMI:
-            return mx ? dispatch_long_semi_max(gap, form, tr, fm, a, la, grid, lds, st)
-                      : dispatch_long_semi_min(gap, form, tr, fm, a, la, grid, lds, st);
+            return mx ? dispatch_long_semi_max(gap, form, tr, fold_max, a, la, grid, lds, st)
+                      : dispatch_long_semi_min(gap, form, tr, fold_max, a, la, grid, lds, st);
     }
     return hipErrorInvalidValue;
 }

@@ -76,11 +76,13 @@ __device__ __forceinline__ constexpr int long_ooff(int w) {
 }
 
 // MX: the strict compare-and-select of every candidate fold done as max
-// (maximising) or min; the host chooses it for score-only solves where it
-// returns the same bits: similarities finite and never -0, and (maximising
-// global problems) gap costs above zero, so that no candidate is -0 or NaN
-// (x + y is -0 only when both are -0) and max / min keep the strict fold's
-// value.
+// (maximising) or min; the host chooses it where it returns the same bits:
+// similarities finite and never -0, and (maximising global problems) gap
+// costs above zero, so that no candidate is -0 or NaN (x + y is -0 only when
+// both are -0) and max / min keep the strict fold's value.  The strict fold
+// then chooses the first candidate in push order equal to that value, so
+// traceback codes come from equality tests beside the value chain instead
+// of lengthening it.
 template <int LOC, int DIR, int GAP, int FORM, int TR, bool MX>
 __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs a, LongArgs la) {
     static_assert(LONG_R == 1, "one row per lane");
@@ -91,7 +93,6 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
     constexpr int DW = long_dw(FORM);
     constexpr float W = worst<DIR>();
     static_assert(FORM == F_TABLE || FORM == F_BINARY, "long mode reads table codes or binary symbols");
-    static_assert(!MX || TR == TR_SCORE, "max folds carry no traceback bits");
 
     extern __shared__ float pa_dyn_lds[];
     // ring[w]: (D, P, symbol bits, -) of the row above compute wave w's
@@ -357,14 +358,30 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                 [[maybe_unused]] uint32_t bits = 0;
                 const float dg = Dd + S;
                 if constexpr (MX) {
+                    float up, lf;  // the vertical and horizontal candidates
                     if constexpr (GAP == GAP_LIN) {
-                        D = opt(opt(dg, inD + go_s), Dl + go_t);
+                        up = inD + go_s;
+                        lf = Dl + go_t;
                     } else {
                         P = opt(inD + go_s, inP + ge_s);
                         Q = opt(Dl + go_t, Ql + ge_t);
-                        D = opt(opt(dg, P), Q);
+                        up = P;
+                        lf = Q;
                     }
+                    D = opt(opt(dg, up), lf);
                     if constexpr (LOC == LOC_LOCAL) D = opt(D, 0.0f);
+                    if constexpr (TR == TR_ONE) {
+                        // the first candidate equal to the fold, in push
+                        // order (local: zero, diagonal, vertical, horizontal)
+                        uint32_t src = up == D ? 2u : 3u;
+                        src = dg == D ? 1u : src;
+                        if constexpr (LOC == LOC_LOCAL) src = D == 0.0f ? 0u : src;
+                        bits = src;
+                        if constexpr (GAP == GAP_AFF) {
+                            bits |= better<DIR>(inP + ge_s, inD + go_s) ? 4u : 0u;
+                            bits |= better<DIR>(Ql + ge_t, Dl + go_t) ? 8u : 0u;
+                        }
+                    }
                 } else if constexpr (GAP == GAP_LIN) {
                     const float up = inD + go_s;
                     const float lf = Dl + go_t;
@@ -567,14 +584,16 @@ static hipError_t dispatch_long_lg(int form, int tr, bool mx, const KArgs &a, co
             if (mx) PA_LONG(F_TABLE, TR_SCORE, true);
             else PA_LONG(F_TABLE, TR_SCORE, false);
         } else {
-            PA_LONG(F_TABLE, TR_ONE, false);
+            if (mx) PA_LONG(F_TABLE, TR_ONE, true);
+            else PA_LONG(F_TABLE, TR_ONE, false);
         }
     } else if (form == F_BINARY) {
         if (tr == TR_SCORE) {
             if (mx) PA_LONG(F_BINARY, TR_SCORE, true);
             else PA_LONG(F_BINARY, TR_SCORE, false);
         } else {
-            PA_LONG(F_BINARY, TR_ONE, false);
+            if (mx) PA_LONG(F_BINARY, TR_ONE, true);
+            else PA_LONG(F_BINARY, TR_ONE, false);
         }
     } else {
         return hipErrorInvalidValue;

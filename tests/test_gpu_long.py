@@ -100,3 +100,28 @@ def test_long_mode_repeats_and_graph_replay():
                                            lim_s=4096, lim_t=2048), S)
         assert np.float32(res[0].score).tobytes() == ref.score.tobytes(), rep
         assert np.array_equal(np.asarray(res[0].path, dtype=np.int32), ref.path()), rep
+
+
+@pytest.mark.parametrize("loc", list(LOCS))
+@pytest.mark.parametrize("neg_zero", [False, True])
+def test_long_mode_fold_choice(loc, neg_zero):
+    """Traceback codes from max folds and equality tests (finite similarities,
+    no -0) and from strict compare-and-select (a -0 in the table) give the
+    oracle's score and path; integer similarities make ties common."""
+    rng = np.random.default_rng(zlib.crc32(repr((loc, neg_zero, "fold")).encode()))
+    ls = np.array([1100], np.int32)
+    lt = np.array([1600], np.int32)
+    cs, ct, os_, ot = ragged_codes(rng, ls, lt, 5)
+    table = sym_table(rng, 5, -2, 3)
+    if neg_zero:
+        table[0, 1] = table[1, 0] = np.float32(-0.0)
+    S = dense_of("indexed", 0, cs, ct, os_, ot, ls, lt, table)
+    for gap in (("linear", 1), ("affine", 3, 1)):
+        solver = make_solver(loc, gap, "maximize", pa.Solution, 2048, 2048)
+        res = solver.solve_ragged("solution", "indexed", ls, lt, codes_s=cs, codes_t=ct, off_s=os_, off_t=ot,
+                                  table=table)
+        assert solver.last_schedule()[1] == 1
+        ref = O.OracleResult(O.make_config(locality=LOCS[loc][1], gap=gap, goal=O.GOAL_ONE, lim_s=2048,
+                                           lim_t=2048), S)
+        assert np.float32(res[0].score).tobytes() == ref.score.tobytes(), gap
+        assert np.array_equal(np.asarray(res[0].path, dtype=np.int32), ref.path()), gap
