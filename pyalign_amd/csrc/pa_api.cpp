@@ -1056,12 +1056,18 @@ void place_plan(Plan &pl, const Input &in, int bits, bool keep_vals, int val_lay
         }
         const int64_t rps = int64_t(pr.L) * pr.R;
         const int64_t nst = (pr.len_s > 0 && pr.len_t > 0) ? (pr.len_s + rps - 1) / rps : 0;
+        pr.pack = 0;
         if (bits && pr.tb_off != NO_TB_OFF) {
             // 16-byte aligned for problems whose steps are whole 64-word
             // rows (the walk loads them four words at a time)
             if (pr.L == WAVE && pr.R == 1) pl.tb_words = (pl.tb_words + 3) & ~int64_t(3);
             pr.tb_off = pl.tb_words;
-            pl.tb_words += nst * (pr.len_t + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
+            if (is_long(pl, pr)) {  // codes packed along the steps
+                pr.pack = long_pack(bits);
+                pl.tb_words += nst * long_strip_words(pr.len_t + WAVE - 1, bits) * WAVE;
+            } else {
+                pl.tb_words += nst * (pr.len_t + pr.L - 1) * pr.L * ((bits * pr.R + 31) / 32);
+            }
         }
         if (nst > 1 && !is_long(pl, pr)) {
             pr.rb_off = pl.rowbuf;

@@ -44,6 +44,7 @@ struct Prob {
     int32_t L;       // lanes cooperating on this problem
     int32_t R;       // rows per lane
     int32_t band;    // band mode: first strip slot (progress, partials); -1 otherwise
+    int32_t pack;    // long mode: traceback codes packed along the steps, long_pack(bits) per word; 0 otherwise
 };
 
 // One wavefront's work: 64/L problems (segment k <- probs[first + k]).
@@ -110,6 +111,18 @@ constexpr int LONG_RING = 64;
 constexpr int long_dw(int form) { return 64 / LONG_T + (form == F_TABLE ? 2 : 1); }
 static_assert((LONG_T == 8 || LONG_T == 16) && LONG_K * LONG_T <= 64, "long-mode phase shape");
 static_assert(long_dw(F_TABLE) < 128 / LONG_T - 1 && LONG_WAVES <= LONG_T, "long-mode ring timing");
+// A long problem's traceback codes (bits per cell: 4 affine, 2 linear),
+// strip by strip, packed along the steps: lane l of the strip of wave w
+// keeps the code of its step s in word (s + b_w) / long_pack(bits) at bit
+// bits * ((s + b_w) % long_pack(bits)); a word index's 64 lanes are
+// consecutive words, a strip's word indexes consecutive (a phase's codes
+// are whole words, stored once per word).
+constexpr int long_pack(int bits) { return 32 / bits < LONG_T ? 32 / bits : LONG_T; }
+// words per lane and strip of steps_per_strip (len_t + 63) steps: the phases
+// of the wave with the largest b_w
+constexpr int64_t long_strip_words(int64_t steps_per_strip, int bits) {
+    return (steps_per_strip + 2 * LONG_T - 2) / LONG_T * (LONG_T / long_pack(bits));
+}
 constexpr uint32_t LONG_SPIN_LIMIT = 1u << 20;  // re-reads before a wait gives up (about a second)
 
 struct LongArgs {

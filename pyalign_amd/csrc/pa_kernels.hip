@@ -444,6 +444,126 @@ __device__ __forceinline__ int walk_rows64(const Prob &pr, const uint32_t *tb, u
     return cnt;
 }
 
+// The walk of a long-mode problem (strips of 64 rows, one row per lane,
+// traceback codes packed along the steps: long_pack(bits) steps per word,
+// see long_strip_words): the moves of walk_step.  Codes come from a tile of
+// whole word rows (64 words: one word index of the strip's lanes) loaded
+// straight into LDS, and from a window of 16 rows x 4 word indexes held one
+// word per lane (lane 4 b + a: row kw - b, word index qw - a) and read with
+// readlane.  The window serves the moves that cannot leave it (`left` more:
+// a move lowers the row by at most one and the step by at most two), about
+// 16 a refill.  Returns the path's length; cells are stored as
+// pa_walk_kernel stores them.
+template <int GAP>
+__device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, uint32_t *lds, int tile_words, int u,
+                                           int v, int32_t *P, int lane) {
+    constexpr int TB = GAP == GAP_AFF ? 4 : 2;
+    constexpr int PK = long_pack(TB);
+    constexpr int LGPK = PK == 16 ? 4 : (PK == 8 ? 3 : 2);
+    static_assert((1 << LGPK) == PK, "steps per word: a power of two");
+    constexpr uint32_t MASK = (1u << TB) - 1u;
+    const int nsp = pr.len_t + WAVE - 1;
+    const int64_t sw = long_strip_words(nsp, TB);
+    // word rows per tile: more than a diagonal path's crossing of a strip
+    const int QT = tile_words / WAVE < 64 ? tile_words / WAVE : 64;
+    int i = u, j = v;
+    int st = i / WAVE;
+    int base = st * WAVE;                            // the strip's first row
+    static_assert((LONG_WAVES & (LONG_WAVES - 1)) == 0, "strip st's wave: st & (LONG_WAVES - 1)");
+    int uq = j + (i - base) + long_boff(st & (LONG_WAVES - 1));  // the step plus the strip's offset
+    int t_st = -1, q0 = 0;  // the tile: word rows [q0, ...] of strip t_st
+    int wl = 0;             // the window's lane of cell (i, j): wl - 4 i - uq / PK
+    int left = -1;          // moves the window still serves
+    uint32_t win = 0;
+    auto refresh = [&]() {
+        if (st != t_st || (uq >> LGPK) < q0) {  // a tile ending at this cell's word row
+            t_st = st;
+            const int q1 = uq >> LGPK;
+            q0 = q1 - QT + 1 > 0 ? q1 - QT + 1 : 0;
+            const uint4 *b4 = reinterpret_cast<const uint4 *>(tb + pr.tb_off + (int64_t(st) * sw + q0) * WAVE);
+            const int n4 = (q1 - q0 + 1) * (WAVE / 4);
+            __builtin_amdgcn_wave_barrier();  // the old tile's reads are done
+            for (int c = 0; c * WAVE < n4; ++c) {
+                if (c * WAVE + lane < n4)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(b4 + c * WAVE + lane),
+                        (__attribute__((address_space(3))) void *)(lds + c * WAVE * 4), 16, 0, 0);
+            }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        const int kw = i - base, qw = uq >> LGPK;
+        const int q = qw - (lane & 3), r = kw - (lane >> 2);
+        win = (q >= q0 && r >= 0) ? lds[(q - q0) * WAVE + r] : 0u;
+        wl = 4 * i + qw;
+        const int lowq = qw - 3 > q0 ? qw - 3 : q0;
+        int m = (uq - lowq * PK) >> 1;
+        m = m < kw ? m : kw;
+        m = m < 15 ? m : 15;
+        left = __builtin_amdgcn_readfirstlane(m);
+    };
+    auto code = [&]() -> uint32_t {
+        if (left < 0) refresh();
+        const uint32_t w = uint32_t(__builtin_amdgcn_readlane(int(win), wl - 4 * i - (uq >> LGPK)));
+        return (w >> ((TB * uq) & 31)) & MASK;  // (TB * (uq % PK))
+    };
+    // one move: di rows up, dj columns left
+    auto move = [&](int di, int dj) {
+        i -= di;
+        j -= dj;
+        uq -= di + dj;
+        --left;
+        if (i < base) {  // into the strip above
+            base -= WAVE;
+            --st;
+            uq = j + WAVE - 1 + long_boff(st & (LONG_WAVES - 1));
+            left = -1;
+        }
+    };
+    int pi = u, pj = v;  // cell 0 sits in lane 0
+    int cnt = 1;
+    auto spill = [&](int upto) {  // cells [upto & ~63, upto)
+        const int b0 = (upto - 1) & ~(WAVE - 1);
+        if (b0 + lane < upto) {
+            P[2 * (b0 + lane)] = pi;
+            P[2 * (b0 + lane) + 1] = pj;
+        }
+    };
+    while ((i | j) >= 0) {
+        uint32_t c = code();
+        const uint32_t src = c & 3u;
+        if (src == 0) break;  // a local start
+        if (src == 1) {
+            move(1, 1);
+        } else if (src == 2) {
+            if constexpr (GAP == GAP_AFF) {
+                while (i > 0 && (c & 4u)) {  // (the first test reads the code already at hand)
+                    move(1, 0);
+                    c = code();
+                }
+            }
+            move(1, 0);
+        } else {
+            if constexpr (GAP == GAP_AFF) {
+                while (j > 0 && (c & 8u)) {
+                    move(0, 1);
+                    c = code();
+                }
+            }
+            move(0, 1);
+        }
+        if (lane == (cnt & (WAVE - 1))) {
+            pi = i;
+            pj = j;
+        }
+        if ((++cnt & (WAVE - 1)) == 0) spill(cnt);
+    }
+    if (cnt & (WAVE - 1)) spill(cnt);
+    return cnt;
+}
+
 template <int GAP>
 __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t n, const uint32_t *tb,
                                                       const int32_t *seeds, int32_t *path,
@@ -464,6 +584,11 @@ __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t
         return;
     }
     int32_t *P = path + 2 * path_off[out];
+    if (pr.pack) {
+        const int cnt = walk_long64<GAP>(pr, tb, smem + wave * tile_words, tile_words, u, v, P, lane);
+        if (lane == 0) path_len[out] = cnt;
+        return;
+    }
     if (pr.L == WAVE && pr.R == 1) {
         const int cnt = walk_rows64<GAP>(pr, tb, smem + wave * tile_words, tile_words, u, v, P, lane);
         if (lane == 0) path_len[out] = cnt;

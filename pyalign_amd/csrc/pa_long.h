@@ -309,7 +309,14 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
         float rbest = W, cbest = W;
         int rj = -1, ci = -1;
         float gscore = W;
-        [[maybe_unused]] uint32_t *tbp = TR == TR_ONE ? a.tb + pr.tb_off + int64_t(st) * nsp * WAVE + l : nullptr;
+        // traceback words: codes of PK steps each (step s at position
+        // (s + bw) % PK of word (s + bw) / PK), a phase's T steps in T / PK
+        // whole words
+        constexpr int TB = GAP == GAP_AFF ? 4 : 2;
+        constexpr int PK = long_pack(TB);
+        [[maybe_unused]] uint32_t *tbp =
+            TR == TR_ONE ? a.tb + pr.tb_off + int64_t(st) * long_strip_words(nsp, TB) * WAVE + l : nullptr;
+        [[maybe_unused]] uint32_t tacc = 0;  // the codes of the current word so far
 
         // the fold of one direction: max / min where exact
         auto opt = [](float x, float y) {
@@ -326,8 +333,8 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
         // (its last phases); NEG, steps before the strip's first step (its
         // first phase)
         constexpr int LO = 1, HI = 2, NEG = 4;
-        // tbq: this lane's traceback word of the phase's first step (step
-        // s0 + t's is tbq[t * WAVE]: constant offsets within a phase)
+        // tbq: this lane's first traceback word of the phase (step s0 + t's
+        // code goes to tbq[(t / PK) * WAVE]: constant offsets within a phase)
         auto body = [&](int s, uint32_t *tbq, auto tc, const float4 &e, const float4 &en, auto checked) {
             [[maybe_unused]] constexpr int t = decltype(tc)::value;
             constexpr int CHK = decltype(checked)::value;
@@ -449,10 +456,13 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
                         if (j == lt - 1) gscore = D;
                     }
                 }
-                // (the words of lanes outside the problem lie in the strip's
-                // area and are never read; steps past the strip have none)
+                // (the words of lanes outside the problem, and the codes of
+                // steps outside the strip, lie in the strip's area and are
+                // never read)
                 if constexpr (TR == TR_ONE) {
-                    if ((CHK & HI) == 0 || s < nsp) tbq[t * WAVE] = bits;
+                    constexpr int r = t % PK;
+                    tacc = r == 0 ? bits : tacc | (bits << (TB * r));
+                    if constexpr (r == PK - 1) tbq[(t / PK) * WAVE] = tacc;
                 }
                 Dl = act ? D : Dl;
                 Ql = act ? Q : Ql;
@@ -473,7 +483,8 @@ __global__ __launch_bounds__((LONG_WAVES + 1) * WAVE) void pa_long_kernel(KArgs 
         // an earlier phase); the first step waits for the first of them
         auto phase_steps = [&](int s0, int obase, auto checked) {
             using std::integral_constant;
-            uint32_t *const tbq = TR == TR_ONE ? tbp + int64_t(s0) * WAVE : nullptr;
+            uint32_t *const tbq = TR == TR_ONE ? tbp + int64_t((s0 + bw) / T) * (T / PK) * WAVE : nullptr;
+            if constexpr (TR == TR_ONE) tacc = 0;  // (steps before the strip's first leave their codes zero)
             float4 cur[T + 1];
             if (l == 0) {
                 const float4 *src = in_ring + ((s0 + bw) & (RING - 1));
