@@ -464,45 +464,68 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
     constexpr uint32_t MASK = (1u << TB) - 1u;
     const int nsp = pr.len_t + WAVE - 1;
     const int64_t sw = long_strip_words(nsp, TB);
-    // word rows per tile: more than a diagonal path's crossing of a strip
-    const int QT = tile_words / WAVE < 64 ? tile_words / WAVE : 64;
+    // two tiles (the current one and the next strip's, loaded ahead) of QT
+    // word rows: more than a diagonal path's crossing of a strip
+    const int QT = tile_words / (2 * WAVE) < 64 ? tile_words / (2 * WAVE) : 64;
     int i = u, j = v;
     int st = i / WAVE;
-    int base = st * WAVE;                            // the strip's first row
+    int base = st * WAVE;  // the strip's first row
     static_assert((LONG_WAVES & (LONG_WAVES - 1)) == 0, "strip st's wave: st & (LONG_WAVES - 1)");
     int uq = j + (i - base) + long_boff(st & (LONG_WAVES - 1));  // the step plus the strip's offset
-    int t_st = -1, q0 = 0;  // the tile: word rows [q0, ...] of strip t_st
-    int wl = 0;             // the window's lane of cell (i, j): wl - 4 i - uq / PK
-    int left = -1;          // moves the window still serves
+    int cur = 0, nxt = QT * WAVE;       // the LDS offsets of the tile read and the one loaded ahead
+    int t_st = -1, q0 = 0;              // cur: word rows [q0, ...] of strip t_st
+    int n_st = -1, nq0 = 0, nq1 = -1;  // nxt: rows [nq0, nq1] of strip n_st
+    int wl = 0;    // the window's lane of cell (i, j): wl - 4 i - uq / PK
+    int left = -1;  // moves the window still serves
     uint32_t win = 0;
+    // loads rows [q1 - QT + 1, q1] (from 0) of strip sq to LDS offset dst, returns the first
+    auto issue = [&](int dst, int sq, int q1) -> int {
+        const int qa = q1 - QT + 1 > 0 ? q1 - QT + 1 : 0;
+        const uint4 *b4 = reinterpret_cast<const uint4 *>(tb + pr.tb_off + (int64_t(sq) * sw + qa) * WAVE);
+        const int n4 = (q1 - qa + 1) * (WAVE / 4);
+        for (int c = 0; c * WAVE < n4; ++c) {
+            if (c * WAVE + lane < n4)
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(b4 + c * WAVE + lane),
+                                                 (__attribute__((address_space(3))) void *)(lds + dst + c * WAVE * 4), 16,
+                                                 0, 0);
+        }
+        return qa;
+    };
     auto refresh = [&]() {
-        if (st != t_st || (uq >> LGPK) < q0) {  // a tile ending at this cell's word row
-            t_st = st;
-            const int q1 = uq >> LGPK;
-            q0 = q1 - QT + 1 > 0 ? q1 - QT + 1 : 0;
-            const uint4 *b4 = reinterpret_cast<const uint4 *>(tb + pr.tb_off + (int64_t(st) * sw + q0) * WAVE);
-            const int n4 = (q1 - q0 + 1) * (WAVE / 4);
-            __builtin_amdgcn_wave_barrier();  // the old tile's reads are done
-            for (int c = 0; c * WAVE < n4; ++c) {
-                if (c * WAVE + lane < n4)
-                    __builtin_amdgcn_global_load_lds(
-                        (const __attribute__((address_space(1))) void *)(b4 + c * WAVE + lane),
-                        (__attribute__((address_space(3))) void *)(lds + c * WAVE * 4), 16, 0, 0);
+        const int qc = uq >> LGPK;
+        if (st != t_st || qc < q0) {  // a tile holding this cell's word row
+            if (st == n_st && qc >= nq0 && qc <= nq1) {  // the one loaded ahead
+                const int t = cur;
+                cur = nxt;
+                nxt = t;
+                q0 = nq0;
+                n_st = -1;
+            } else {
+                __builtin_amdgcn_wave_barrier();  // the old tile's reads are done
+                q0 = issue(cur, st, qc);
             }
+            t_st = st;
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        const int kw = i - base, qw = uq >> LGPK;
-        const int q = qw - (lane & 3), r = kw - (lane >> 2);
-        win = (q >= q0 && r >= 0) ? lds[(q - q0) * WAVE + r] : 0u;
-        wl = 4 * i + qw;
-        const int lowq = qw - 3 > q0 ? qw - 3 : q0;
+        const int kw = i - base;
+        const int q = qc - (lane & 3), r = kw - (lane >> 2);
+        win = (q >= q0 && r >= 0) ? lds[cur + (q - q0) * WAVE + r] : 0u;
+        wl = 4 * i + qc;
+        const int lowq = qc - 3 > q0 ? qc - 3 : q0;
         int m = (uq - lowq * PK) >> 1;
         m = m < kw ? m : kw;
         m = m < 15 ? m : 15;
         left = __builtin_amdgcn_readfirstlane(m);
+        // near the strip's top: the strip above, rows up to the path's
+        // latest possible entry (column j, row 63), loaded ahead
+        if (kw < 32 && st > 0 && n_st != st - 1) {
+            n_st = st - 1;
+            nq1 = (j + WAVE - 1 + long_boff(n_st & (LONG_WAVES - 1))) >> LGPK;
+            nq0 = issue(nxt, n_st, nq1);
+        }
     };
     auto code = [&]() -> uint32_t {
         if (left < 0) refresh();
