@@ -1225,9 +1225,12 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             }
         }
     }
+    // long mode packs its traceback codes along the steps, a layout only the
+    // walk of a solve reads: a tape's backward walks the per-lane layout
+    const bool allow_long = !keep_vals && !tape;
     const std::vector<int64_t> key = {n, job.loc, job.dir, job.gap, job.form, job.tr, int64_t(job.fast),
                                       in.schedule, in.dense_si_fixed, in.dense_sj_fixed, s.cus,
-                                      int64_t(in.matrices), n_ckpt ? s.ckpt_min : -1};
+                                      int64_t(in.matrices), n_ckpt ? s.ckpt_min : -1, int64_t(allow_long)};
     PlanCache &pc = s.last_plan;
     auto same = [&]() {
         if (!pc.valid || pc.key != key) return false;
@@ -1243,7 +1246,7 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
         pc.valid = false;
         pc.plan = in.schedule == PA_SCHED_PLAIN ? plan_plain(job, in.len_s, in.len_t, n)
                                                 : plan_auto(job, in.len_s, in.len_t, n, s.cus * 4,
-                                                            n_ckpt ? ckpt.data() : nullptr, true, !keep_vals);
+                                                            n_ckpt ? ckpt.data() : nullptr, true, allow_long);
         place_plan(pc.plan, in, bits, keep_vals, val_layers);
         pc.uploaded = false;
         pc.key = key;

@@ -125,3 +125,40 @@ def test_long_mode_fold_choice(loc, neg_zero):
                                            lim_t=2048), S)
         assert np.float32(res[0].score).tobytes() == ref.score.tobytes(), gap
         assert np.array_equal(np.asarray(res[0].path, dtype=np.int32), ref.path()), gap
+
+
+@pytest.mark.parametrize("loc", ["local", "global"])
+def test_long_pair_tape_gap_gradients(loc):
+    """A tape of a long binary pair (pa_forward_ragged, gap gradients only)
+    runs outside long mode, whose step-packed codes only a solve's walk
+    reads: its backward walks the per-lane layout, and the gap gradients
+    count the gap runs of the oracle's path."""
+    import ctypes
+
+    import torch
+
+    from pyalign_amd import algorithm
+    from pyalign_amd._lib import check, lib
+    from tests.test_gpu_autograd import gap_counts
+    rng = np.random.default_rng(zlib.crc32(repr((loc, "tape")).encode()))
+    ls = np.array([2300], np.int32)
+    lt = np.array([1900], np.int32)
+    cs, ct, os_, ot = ragged_codes(rng, ls, lt, 4)
+    solver = make_solver(loc, ("affine", 3, 1), "maximize", pa.Solution, 4096, 2048)
+    rg, keep = algorithm.make_ragged("binary", ls, lt, codes_s=cs.astype(np.uint32), codes_t=ct.astype(np.uint32),
+                                     off_s=os_, off_t=ot, eq=1.0, ne=-1.0)
+    scores = torch.empty(1, device="cuda")
+    h = ctypes.c_void_p()
+    check(lib().pa_forward_ragged(solver._h, ctypes.byref(rg), ctypes.c_void_p(scores.data_ptr()), ctypes.byref(h)))
+    check(lib().pa_solver_sync(solver._h))
+    g = torch.ones(1, device="cuda")
+    gg = torch.zeros(2, device="cuda")
+    check(lib().pa_backward(h, ctypes.c_void_p(g.data_ptr()), None, None, None, ctypes.c_void_p(gg.data_ptr())))
+    lib().pa_tape_free(h)
+    S = dense_of("binary", 0, cs, ct, os_, ot, ls, lt, eq=1.0, ne=-1.0)
+    ref = O.OracleResult(O.make_config(locality=LOCS[loc][1], gap=("affine", 3, 1), goal=O.GOAL_ONE,
+                                       lim_s=4096, lim_t=2048), S)
+    assert np.float32(scores[0].item()).tobytes() == ref.score.tobytes()
+    runs, ext = gap_counts(ref.path().tolist(), LOCS[loc][1])
+    assert gg.cpu().tolist() == [-float(runs), -float(ext)]
+    del keep
