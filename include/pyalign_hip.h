@@ -247,13 +247,14 @@ int pa_solver_set_stream(pa_solver *solver, void *stream, int32_t external);
  * by pa_solver_set_stream), so that a caller can order events or work of its
  * own with the solver's.  No reference counterpart. */
 int pa_solver_stream(const pa_solver *solver, void **stream);
-/* Checkpointed differentiation: in pa_forward_ragged, a global problem on a
- * device-resident dense or series payload whose longer side exceeds min_len
- * (default 2048) keeps only the bottom row of each strip of 512 rows, not
- * its traceback; pa_backward recomputes its traceback from those rows into
- * a scratch buffer of the solver (whole paths where a quarter of the free
- * device memory holds them, else one strip at a time, last strip first)
- * and walks it.  Negative: never.  No reference counterpart. */
+/* Checkpointed differentiation: in pa_forward_ragged, a problem (any
+ * locality) on a device-resident dense or series payload whose longer side
+ * exceeds min_len (default 2048) keeps only the bottom row of each strip of
+ * 512 rows and the path's end (its seed), not its traceback; pa_backward
+ * recomputes its traceback from those rows into a scratch buffer of the
+ * solver (whole paths where a quarter of the free device memory holds them,
+ * else one strip at a time, from the seed's strip up) and walks it.
+ * Negative: never.  No reference counterpart. */
 int pa_solver_set_checkpoint(pa_solver *solver, int64_t min_len);
 /* Where the last solve from raw symbols met symbols outside the alphabet:
  * the first such position in s's and in t's symbol array, -1 for none.  No

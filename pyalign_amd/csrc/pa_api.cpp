@@ -1209,12 +1209,13 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     const int bits = tb_bits(job.gap, job.tr);
     const int64_t *o_s = in.form == PA_FORM_DENSE ? in.dense_off : in.off_s;
     const int64_t *o_t = in.form == PA_FORM_DENSE ? nullptr : in.off_t;
-    // checkpointed forward (a tape of global problems on a device payload):
-    // pairs longer than the solver's threshold keep only their strip rows
+    // checkpointed forward (a tape of problems on a device payload): pairs
+    // longer than the solver's threshold keep only their strip rows (and,
+    // local and semiglobal, the seed the forward found)
     std::vector<uint8_t> ckpt;
     int64_t n_ckpt = 0;
-    if (tape && job.loc == LOC_GLOBAL && (job.form == F_DENSE || job.form == F_SERIES) && !in.matrices &&
-        s.ckpt_min >= 0 && in.schedule == PA_SCHED_AUTO) {
+    if (tape && (job.form == F_DENSE || job.form == F_SERIES) && !in.matrices && s.ckpt_min >= 0 &&
+        in.schedule == PA_SCHED_AUTO) {
         ckpt.assign(size_t(n), 0);
         const int64_t rps = int64_t(BAND_L) * BAND_R;
         for (int64_t k = 0; k < n; k++) {
@@ -2329,8 +2330,9 @@ namespace {
 // whole recomputed traceback fits the scratch budget are done in chunks:
 // every strip of the chunk recomputed in one launch, then the ordinary
 // backward walk over them.  A problem too large for that walks one strip per
-// pass, last strip first: each pass recomputes that strip into the scratch
-// buffer and pa_backward_strip_kernel continues the walk through it.  The
+// pass, from the strip of its seed (the path's end the forward found) up:
+// each pass recomputes that strip into the scratch buffer and
+// pa_backward_strip_kernel continues the walk through it.  The
 // scratch buffer belongs to the solver and is reused by its tapes.
 void backward_checkpointed(pa_tape &t, const BwArgs &base, int form, hipStream_t st) {
     pa_solver &s = *t.solver;
@@ -2507,29 +2509,36 @@ void backward_checkpointed(pa_tape &t, const BwArgs &base, int form, hipStream_t
         const int nc = int(c1 - c0);
         std::vector<int64_t> base_off(static_cast<size_t>(nc));
         std::vector<StripWalk> init(static_cast<size_t>(nc));
-        int max_nst = 0;
+        std::vector<int> first(static_cast<size_t>(nc));  // the seed's strip: the walk's first
+        int passes = 0;
         int64_t acc = 0;
         for (int q = 0; q < nc; q++) {
             const Prob &pr = by_strip[c0 + size_t(q)];
             base_off[size_t(q)] = acc;
             acc += words_of(pr);
-            max_nst = std::max(max_nst, strips_of(pr));
+            int32_t seed[2];  // the forward's (global: the last cell)
+            HIPCHK(hipMemcpyAsync(seed, base.seeds + 2 * pr.out, 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
             StripWalk w;
             std::memset(&w, 0, sizeof(w));
-            w.i = pr.len_s - 1;  // the global seed
-            w.j = pr.len_t - 1;
+            w.i = seed[0];
+            w.j = seed[1];
             w.cur_i = w.cur_j = -1;
             init[size_t(q)] = w;
+            // no seed (a local problem with no cell above zero): one pass
+            // that finds the walk finished and adds nothing
+            first[size_t(q)] = seed[0] >= 0 && seed[1] >= 0 ? seed[0] / (pr.L * pr.R) : 0;
+            passes = std::max(passes, first[size_t(q)] + 1);
         }
         s.b_ckpt_tb.ensure(size_t(std::max<int64_t>(acc, 1)) * 4);
         upload(st, t.s_state, init.data(), init.size());
-        for (int it = 0; it < max_nst; it++) {
+        for (int it = 0; it < passes; it++) {
             std::vector<Prob> probs(by_strip.begin() + int64_t(c0), by_strip.begin() + int64_t(c1));
             std::vector<int32_t> strip_of(static_cast<size_t>(nc), -1);
             std::vector<Group> groups;
             for (int q = 0; q < nc; q++) {
                 Prob &pr = probs[size_t(q)];
-                const int sp = strips_of(pr) - 1 - it;
+                const int sp = first[size_t(q)] - it;
                 if (sp < 0) continue;
                 // the strip's words land at this problem's scratch slot
                 pr.tb_off = base_off[size_t(q)] - int64_t(sp) * words_of(pr);

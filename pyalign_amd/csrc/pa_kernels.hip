@@ -945,7 +945,7 @@ hipError_t launch_band_combine(int loc, int dir, int tr, const Prob *probs, cons
 
 // ---------------------------------------------------------------------------
 // Checkpointed backward: the walk of pa_backward_kernel continued one strip
-// at a time (last strip first), on a traceback recomputed for that strip
+// at a time (from the seed's strip up), on a traceback recomputed for that strip
 // only.  The walk stops when it leaves the strip upwards and saves where it
 // is, including a vertical gap run still searching for its origin (the
 // only move that crosses strips), the gap counts so far and the DTW

@@ -150,9 +150,10 @@ def align_score(S, len_s, len_t, gap_open=1.0, gap_extend=0.2, locality="global"
 
     gap_open / gap_extend may be floats or 0-dim tensors (requires_grad for
     gap gradients).  Returns float32 scores [N], identical to the solvers'
-    ``Score`` results on the same inputs.  Global pairs whose longer side
-    exceeds ``checkpoint`` keep only strip boundary rows in the forward and
-    are recomputed strip by strip in the backward (-1: never).
+    ``Score`` results on the same inputs.  Pairs whose longer side exceeds
+    ``checkpoint`` keep only strip boundary rows (and the path's end the
+    forward found) and are recomputed strip by strip in the backward (-1:
+    never).
     """
     if gap not in ("affine", "linear"):
         raise ValueError(gap)

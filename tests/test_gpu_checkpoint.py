@@ -1,9 +1,9 @@
 """Checkpointed differentiation on the GPU.
 
-Long global pairs keep only strip boundary rows in the forward; the
-backward recomputes their tracebacks from those rows: whole paths at once
-where the scratch budget allows, else one strip at a time (last strip
-first), continuing the walk through each.  Scores and every gradient must be
+Long pairs keep only strip boundary rows (and the path's end) in the
+forward; the backward recomputes their tracebacks from those rows: whole
+paths at once where the scratch budget allows, else one strip at a time
+(from the seed's strip up), continuing the walk through each.  Scores and every gradient must be
 bit-identical to the run that keeps the full traceback, with the default
 budget and with a budget small enough to force the strip-by-strip passes,
 and the schedule must show the pairs checkpointed.
@@ -47,7 +47,7 @@ def dtw_run(X, Y, lx, ly, checkpoint):
 # for a few whole paths at a time (chunks recomputed while the previous
 # chunk's walk runs on a second stream), or less than the whole traceback of
 # the longer pairs (those walk strip by strip)
-SCRATCH = {"auto": None, "chunks": "1000000", "strips": "300000"}
+SCRATCH = {"auto": None, "chunks": "1000000", "strips": "150000"}
 
 
 def with_scratch(monkeypatch, scratch, fn):
@@ -72,11 +72,11 @@ def test_dtw_checkpointed_equals_full_traceback(dim, scratch, monkeypatch):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-def align_run(S, ls, lt, checkpoint, gap):
+def align_run(S, ls, lt, checkpoint, gap, locality="global"):
     St = torch.tensor(S, device="cuda", requires_grad=True)
     go = torch.tensor(1.0, requires_grad=True)
     ge = torch.tensor(0.2, requires_grad=True)
-    s = align_score(St, ls, lt, go, ge if gap == "affine" else 0.0, locality="global", gap=gap,
+    s = align_score(St, ls, lt, go, ge if gap == "affine" else 0.0, locality=locality, gap=gap,
                     checkpoint=checkpoint)
     sched = last_schedule()
     s.sum().backward()
@@ -88,15 +88,41 @@ def align_run(S, ls, lt, checkpoint, gap):
 
 @pytest.mark.parametrize("scratch", list(SCRATCH))
 @pytest.mark.parametrize("gap", ["affine", "linear"])
-def test_align_checkpointed_equals_full_traceback(gap, scratch, monkeypatch):
+@pytest.mark.parametrize("locality", ["global", "local", "semiglobal"])
+def test_align_checkpointed_equals_full_traceback(locality, gap, scratch, monkeypatch):
+    """Local and semiglobal paths end where the forward's seed search put
+    them (inside the matrix, or on its last row or column): the strip walk
+    starts in the seed's strip."""
     rng = np.random.default_rng(111)
     n = 8
     ls = rng.integers(200, 1400, n).astype(np.int32)
     lt = rng.integers(200, 1400, n).astype(np.int32)
     S = (rng.standard_normal((n, int(ls.max()), int(lt.max()))) / 4).astype(np.float32)
-    full, sf = align_run(S, ls, lt, -1, gap)
-    ck, sc = with_scratch(monkeypatch, scratch, lambda: align_run(S, ls, lt, 600, gap))
+    if locality == "local":
+        S -= np.float32(0.1)  # local paths shorter than the matrix, ending inside it
+    if scratch == "strips":  # some checkpointed pairs walk strip by strip: traceback words above the scratch
+        ck_pairs = (np.maximum(ls, lt) > 600) & (ls > 512)
+        words = (lt.astype(np.int64) + 63) * 64 * ((ls + 511) // 512)
+        assert (words[ck_pairs] > int(SCRATCH[scratch])).sum() >= 2
+    full, sf = align_run(S, ls, lt, -1, gap, locality)
+    ck, sc = with_scratch(monkeypatch, scratch, lambda: align_run(S, ls, lt, 600, gap, locality))
     assert sf[2] == 0 and sc[2] > 0, (sf, sc)
+    for a, b in zip(full, ck):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_local_checkpointed_without_a_positive_cell(monkeypatch):
+    """A long local pair whose similarities are all negative has no path (no
+    seed): its checkpointed backward adds nothing, strip by strip too."""
+    rng = np.random.default_rng(113)
+    ls = np.array([1500, 300], np.int32)
+    lt = np.array([1800, 250], np.int32)  # 3 strips of 1862 x 64 words: more than the scratch
+    S = (rng.standard_normal((2, 1500, 1800)) / 4).astype(np.float32)
+    S[0] = -np.abs(S[0]) - np.float32(0.5)
+    full, _ = align_run(S, ls, lt, -1, "affine", "local")
+    ck, sc = with_scratch(monkeypatch, "strips", lambda: align_run(S, ls, lt, 600, "affine", "local"))
+    assert sc[2] > 0
+    assert full[0][0] == 0.0 and not full[1][0].any()
     for a, b in zip(full, ck):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
