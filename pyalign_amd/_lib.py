@@ -107,6 +107,14 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                 "(hipcc for gfx950); pyalign_amd has no CPU fallback")
+        # One HIP runtime per process: torch's build bundles its own (same
+        # soname, loaded through a different name), and a process that loads
+        # this library's first cannot open the GPU from torch afterwards.
+        # Loading torch first makes this library bind to torch's copy.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -446,13 +446,18 @@ __device__ __forceinline__ int walk_rows64(const Prob &pr, const uint32_t *tb, u
 
 // The walk of a long-mode problem (strips of 64 rows, one row per lane,
 // traceback codes packed along the steps: long_pack(bits) steps per word,
-// see long_strip_words): the moves of walk_step.  Codes come from a tile of
-// whole word rows (64 words: one word index of the strip's lanes) loaded
-// straight into LDS, and from a window of 16 rows x 4 word indexes held one
-// word per lane (lane 4 b + a: row kw - b, word index qw - a) and read with
-// readlane.  The window serves the moves that cannot leave it (`left` more:
-// a move lowers the row by at most one and the step by at most two), about
-// 16 a refill.  Returns the path's length; cells are stored as
+// see long_strip_words): the moves of walk_step, taken a run at a time.
+// Lane m reads the code of the cell m moves ahead along the run the path
+// is in (a diagonal run: m rows up and m columns left; an affine gap run:
+// m cells up or left), and a ballot of "the run goes on there" gives the
+// run's length: one LDS round trip for up to 64 moves instead of one per
+// move.  A diagonal run continues while the cells' codes say diagonal; a
+// gap run (one path move, solver.h:2999-3016) while their extend bits are
+// set.  Runs stop at the strip's top and at the tile's bottom and resume
+// from there.  Codes come from a tile of whole word rows (64 words: one
+// word index of the strip's lanes) loaded straight into LDS; the next
+// strip's tile is loaded ahead into a second buffer once the path is in the
+// strip's top half.  Returns the path's length; cells are stored as
 // pa_walk_kernel stores them.
 template <int GAP>
 __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, uint32_t *lds, int tile_words, int u,
@@ -475,9 +480,6 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
     int cur = 0, nxt = QT * WAVE;       // the LDS offsets of the tile read and the one loaded ahead
     int t_st = -1, q0 = 0;              // cur: word rows [q0, ...] of strip t_st
     int n_st = -1, nq0 = 0, nq1 = -1;  // nxt: rows [nq0, nq1] of strip n_st
-    int wl = 0;    // the window's lane of cell (i, j): wl - 4 i - uq / PK
-    int left = -1;  // moves the window still serves
-    uint32_t win = 0;
     // loads rows [q1 - QT + 1, q1] (from 0) of strip sq to LDS offset dst, returns the first
     auto issue = [&](int dst, int sq, int q1) -> int {
         const int qa = q1 - QT + 1 > 0 ? q1 - QT + 1 : 0;
@@ -491,7 +493,54 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
         }
         return qa;
     };
-    auto refresh = [&]() {
+    int pi = u, pj = v;  // cell 0 sits in lane 0
+    int cnt = 1;
+    auto spill = [&](int upto) {  // cells [upto & ~63, upto)
+        const int b0 = (upto - 1) & ~(WAVE - 1);
+        if (b0 + lane < upto) {
+            P[2 * (b0 + lane)] = pi;
+            P[2 * (b0 + lane) + 1] = pj;
+        }
+    };
+    // path cells cnt .. cnt + n - 1: (i0 - di (t + 1), j0 - dj (t + 1)), t < n
+    auto record = [&](int n, int i0, int j0, int di, int dj) {
+        int t0 = 0;
+        while (t0 < n) {
+            const int slot = cnt & (WAVE - 1);
+            const int take = n - t0 < WAVE - slot ? n - t0 : WAVE - slot;
+            const int t = t0 + (lane - slot) + 1;
+            if (lane >= slot && lane < slot + take) {
+                pi = i0 - di * t;
+                pj = j0 - dj * t;
+            }
+            cnt += take;
+            t0 += take;
+            if ((cnt & (WAVE - 1)) == 0) spill(cnt);
+        }
+    };
+    // the code of the cell dk * lane rows up and du * lane steps back (for
+    // this lane), and whether that cell lies in the strip and the tile
+    auto gather = [&](int dk, int du, bool &ok) -> uint32_t {
+        const int r = (i - base) - dk * lane;
+        const int uu = uq - du * lane;
+        const int q = uu >> LGPK;
+        ok = r >= 0 && q >= q0;
+        const uint32_t w = ok ? lds[cur + (q - q0) * WAVE + r] : 0u;
+        return (w >> ((TB * uu) & 31)) & MASK;
+    };
+    // a run of n moves of di rows and dj columns (the last one may leave the strip)
+    auto advance = [&](int n, int di, int dj) {
+        i -= di * n;
+        j -= dj * n;
+        uq -= (di + dj) * n;
+        if (i < base) {  // into the strip above
+            base -= WAVE;
+            --st;
+            uq = j + (i - base) + long_boff(st & (LONG_WAVES - 1));
+        }
+    };
+    int mode = 0;  // 0: at a cell; 2 / 3: inside a vertical / horizontal gap run (affine)
+    while ((i | j) >= 0) {
         const int qc = uq >> LGPK;
         if (st != t_st || qc < q0) {  // a tile holding this cell's word row
             if (st == n_st && qc >= nq0 && qc <= nq1) {  // the one loaded ahead
@@ -509,79 +558,57 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // in the strip's top half: the strip above, rows up to the path's
+            // latest possible entry (column j, row 63), loaded ahead
         }
-        const int kw = i - base;
-        const int q = qc - (lane & 3), r = kw - (lane >> 2);
-        win = (q >= q0 && r >= 0) ? lds[cur + (q - q0) * WAVE + r] : 0u;
-        wl = 4 * i + qc;
-        const int lowq = qc - 3 > q0 ? qc - 3 : q0;
-        int m = (uq - lowq * PK) >> 1;
-        m = m < kw ? m : kw;
-        m = m < 15 ? m : 15;
-        left = __builtin_amdgcn_readfirstlane(m);
-        // near the strip's top: the strip above, rows up to the path's
-        // latest possible entry (column j, row 63), loaded ahead
-        if (kw < 32 && st > 0 && n_st != st - 1) {
+        if (i - base < WAVE / 2 && st > 0 && n_st != st - 1) {
             n_st = st - 1;
             nq1 = (j + WAVE - 1 + long_boff(n_st & (LONG_WAVES - 1))) >> LGPK;
             nq0 = issue(nxt, n_st, nq1);
         }
-    };
-    auto code = [&]() -> uint32_t {
-        if (left < 0) refresh();
-        const uint32_t w = uint32_t(__builtin_amdgcn_readlane(int(win), wl - 4 * i - (uq >> LGPK)));
-        return (w >> ((TB * uq) & 31)) & MASK;  // (TB * (uq % PK))
-    };
-    // one move: di rows up, dj columns left
-    auto move = [&](int di, int dj) {
-        i -= di;
-        j -= dj;
-        uq -= di + dj;
-        --left;
-        if (i < base) {  // into the strip above
-            base -= WAVE;
-            --st;
-            uq = j + WAVE - 1 + long_boff(st & (LONG_WAVES - 1));
-            left = -1;
-        }
-    };
-    int pi = u, pj = v;  // cell 0 sits in lane 0
-    int cnt = 1;
-    auto spill = [&](int upto) {  // cells [upto & ~63, upto)
-        const int b0 = (upto - 1) & ~(WAVE - 1);
-        if (b0 + lane < upto) {
-            P[2 * (b0 + lane)] = pi;
-            P[2 * (b0 + lane) + 1] = pj;
-        }
-    };
-    while ((i | j) >= 0) {
-        uint32_t c = code();
-        const uint32_t src = c & 3u;
-        if (src == 0) break;  // a local start
-        if (src == 1) {
-            move(1, 1);
-        } else if (src == 2) {
-            if constexpr (GAP == GAP_AFF) {
-                while (i > 0 && (c & 4u)) {  // (the first test reads the code already at hand)
-                    move(1, 0);
-                    c = code();
+        if (mode == 0) {
+            // a diagonal run: lane m, the cell m moves on (inside the matrix: j - m >= 0)
+            bool ok;
+            const uint32_t c = gather(1, 2, ok);
+            ok = ok && j - lane >= 0;
+            const uint64_t run = __ballot(ok && (c & 3u) == 1u);
+            const int n = ~run == 0 ? WAVE : __builtin_ctzll(~run);
+            const uint64_t valid = __ballot(ok);
+            if (n > 0) {
+                record(n, i, j, 1, 1);
+                advance(n, 1, 1);
+            }
+            // the cell the run stopped at, when it lies in this strip and tile
+            if (n < WAVE && ((valid >> n) & 1u)) {
+                const uint32_t cn = uint32_t(__builtin_amdgcn_readlane(int(c), n));
+                const uint32_t src = cn & 3u;
+                if (src == 0) break;  // a local start
+                if constexpr (GAP == GAP_AFF) {
+                    mode = int(src);  // the gap run starts at this cell
+                } else {
+                    record(1, i, j, src == 2 ? 1 : 0, src == 3 ? 1 : 0);
+                    advance(1, src == 2 ? 1 : 0, src == 3 ? 1 : 0);
                 }
             }
-            move(1, 0);
         } else {
-            if constexpr (GAP == GAP_AFF) {
-                while (j > 0 && (c & 8u)) {
-                    move(0, 1);
-                    c = code();
-                }
+            // a gap run: lane m, the cell m moves on; the run goes on while
+            // the extend bit is set away from the border (solver.h:2999-3016)
+            const bool vert = mode == 2;
+            bool ok;
+            const uint32_t c = gather(vert ? 1 : 0, 1, ok);
+            const int pos = (vert ? i : j) - lane;
+            const uint64_t ext = __ballot(ok && pos > 0 && (c & (vert ? 4u : 8u)) != 0u);
+            const int n = ~ext == 0 ? WAVE : __builtin_ctzll(~ext);
+            const uint64_t valid = __ballot(ok);
+            if (n < WAVE && ((valid >> n) & 1u)) {  // the run's last cell: one more move ends it
+                const int di = vert ? 1 : 0, dj = vert ? 0 : 1;
+                record(1, i - di * n, j - dj * n, di, dj);
+                advance(n + 1, di, dj);
+                mode = 0;
+            } else {  // the run goes on past this strip or tile
+                advance(n, vert ? 1 : 0, vert ? 0 : 1);
             }
-            move(0, 1);
         }
-        if (lane == (cnt & (WAVE - 1))) {
-            pi = i;
-            pj = j;
-        }
-        if ((++cnt & (WAVE - 1)) == 0) spill(cnt);
     }
     if (cnt & (WAVE - 1)) spill(cnt);
     return cnt;
