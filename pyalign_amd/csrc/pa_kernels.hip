@@ -167,6 +167,7 @@ struct TbWalker {
     int kww;  // words per tile step: (k1 - k0) * wpc
     int i, j, st, k, bit;
     int s, li;
+    int lgR, lgrps;  // log2 R and log2 (L R): strip heights and lane counts are powers of two
     int64_t ga, gsi, gsj;
 
     __device__ void init(uint32_t *l, int tile_words, const uint32_t *t, const Prob &pr, int b) {
@@ -177,6 +178,8 @@ struct TbWalker {
         R = pr.R;
         bits = b;
         wpc = (b * R + 31) / 32;
+        lgR = __builtin_ctz(unsigned(R));
+        lgrps = __builtin_ctz(unsigned(L * R));
         nsteps = pr.len_t + L - 1;
         // tile shape: a path going diagonally leaves a lane every R rows, one
         // step more than R, so tiles hold about R + 1 steps per lane
@@ -227,6 +230,32 @@ struct TbWalker {
         --s;
         li -= kww;
         ga -= gsj;
+    }
+    // moves to (ni, nj), up and left of the current cell, keeping the tile
+    // (the next code() loads another when the cell lies outside it)
+    __device__ void jump(int ni, int nj) {
+        i = ni;
+        j = nj;
+        st = i >> lgrps;  // (-1 above the matrix)
+        const int rem = i & ((1 << lgrps) - 1);
+        k = rem >> lgR;
+        bit = (rem & (R - 1)) * bits;
+        s = j + k;
+        ga = int64_t(i) * gsi + int64_t(j) * gsj;
+        li = (s - s0) * kww + (k - k0) * wpc;
+    }
+    // this lane's cell of a run: lane m, the cell m moves of di rows up and
+    // dj columns left from the current one; ok when it lies in the matrix
+    // and the loaded tile (which holds the current cell)
+    __device__ uint32_t gather(int di, int dj, int lane, bool &ok) const {
+        const int ii = i - di * lane, jj = j - dj * lane;
+        const int rem = ii & ((1 << lgrps) - 1);
+        const int k2 = rem >> lgR;
+        const int s2 = jj + k2;
+        ok = ii >= 0 && jj >= 0 && (ii >> lgrps) == t_st && k2 >= k0 && s2 >= s0;
+        const int b2 = (rem & (R - 1)) * bits;
+        const uint32_t wd = ok ? lds[(s2 - s0) * kww + (k2 - k0) * wpc + (b2 >> 5)] : 0u;
+        return (wd >> (b2 & 31)) & ((1u << bits) - 1u);
     }
     // traceback code of the current cell (i, j >= 0; the same in every lane)
     __device__ uint32_t code(int lane) {
@@ -822,19 +851,74 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_wor
     int u = u0, v = v0;
     if (u0 != NO_SEED) {
         w.seek(u0, v0);
-        while (w.i >= 0 && w.j >= 0) {
-            const int pu = w.i, pv = w.j;
-            const int64_t pga = w.ga;
-            const int src = walk_step<GAP>(w, lane);
-            if constexpr (GAP == GAP_DTW) cost_grad(pu, pv, pga);
-            if (src == 0) break;  // local: the path starts at a zero cell
-            if (src == 1) {
-                if constexpr (GAP != GAP_DTW) cost_grad(pu, pv, pga);
-            } else if constexpr (GAP == GAP_AFF) {
-                opens += 1.0f;
-                extends += float((pu - w.i) + (pv - w.j) - 1);  // run length - 1
-            } else if constexpr (GAP == GAP_LIN) {
-                opens += 1.0f;  // linear: one gap symbol per move
+        if constexpr (FORM == F_DENSE) {
+            // A run at a time: lane m reads the cell m moves on along the run
+            // the path is in (cells of one code: diagonal moves, and linear gap
+            // or DTW moves of one direction; or an affine gap run, whose cells
+            // extend it while their extend bit is set), a ballot gives the run's
+            // length, and the run's gradient cells (each touched once) go out as
+            // one atomic instruction.  The moves are walk_step's.
+            int mode = 0, pu = 0;  // affine: 2 / 3 inside a vertical / horizontal gap run from row / column pu
+            while (w.i >= 0 && w.j >= 0) {
+                const uint32_t c0 = w.code(lane);  // (loads a tile holding the current cell)
+                if (mode == 0) {
+                    const int src = int(c0 & 3u);
+                    if (src == 0) {  // local: the path starts at a zero cell
+                        if constexpr (GAP == GAP_DTW) cost_grad(w.i, w.j, w.ga);
+                        break;
+                    }
+                    if constexpr (GAP == GAP_AFF) {
+                        if (src >= 2) {  // a gap run starts at this cell
+                            mode = src;
+                            pu = src == 2 ? w.i : w.j;
+                            continue;
+                        }
+                    }
+                    const int di = src != 3 ? 1 : 0, dj = src != 2 ? 1 : 0;
+                    bool ok;
+                    const uint32_t cm = w.gather(di, dj, lane, ok);
+                    const uint64_t run = __ballot(ok && int(cm & 3u) == src);
+                    const int n = ~run == 0 ? WAVE : __builtin_ctzll(~run);  // (at least 1: the current cell)
+                    if (GAP == GAP_DTW || src == 1) {
+                        if (a.grad_dense && lane < n)
+                            add_noret(a.grad_dense + pr.off_s + w.ga - int64_t(lane) * (di * w.gsi + dj * w.gsj), g);
+                    } else {
+                        opens += float(n);  // linear: one gap symbol per move
+                    }
+                    w.jump(w.i - di * n, w.j - dj * n);
+                } else {
+                    const bool vert = mode == 2;
+                    bool ok;
+                    const uint32_t cm = w.gather(vert ? 1 : 0, vert ? 0 : 1, lane, ok);
+                    const int pos = (vert ? w.i : w.j) - lane;
+                    const uint64_t ext = __ballot(ok && pos > 0 && (cm & (vert ? 4u : 8u)) != 0u);
+                    const int n = ~ext == 0 ? WAVE : __builtin_ctzll(~ext);
+                    const uint64_t valid = __ballot(ok);
+                    if (n < WAVE && ((valid >> n) & 1u)) {  // the run's last cell: one more move ends it
+                        w.jump(w.i - (vert ? n + 1 : 0), w.j - (vert ? 0 : n + 1));
+                        opens += 1.0f;
+                        extends += float(pu - (vert ? w.i : w.j) - 1);  // run length - 1
+                        mode = 0;
+                    } else {  // the run goes on past the tile
+                        w.jump(w.i - (vert ? n : 0), w.j - (vert ? 0 : n));
+                    }
+                }
+            }
+        } else {
+            while (w.i >= 0 && w.j >= 0) {
+                const int pu = w.i, pv = w.j;
+                const int64_t pga = w.ga;
+                const int src = walk_step<GAP>(w, lane);
+                if constexpr (GAP == GAP_DTW) cost_grad(pu, pv, pga);
+                if (src == 0) break;  // local: the path starts at a zero cell
+                if (src == 1) {
+                    if constexpr (GAP != GAP_DTW) cost_grad(pu, pv, pga);
+                } else if constexpr (GAP == GAP_AFF) {
+                    opens += 1.0f;
+                    extends += float((pu - w.i) + (pv - w.j) - 1);  // run length - 1
+                } else if constexpr (GAP == GAP_LIN) {
+                    opens += 1.0f;  // linear: one gap symbol per move
+                }
             }
         }
         u = w.i;
