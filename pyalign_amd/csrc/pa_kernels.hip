@@ -688,13 +688,61 @@ __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t
             P[2 * (base + lane) + 1] = pj;
         }
     };
-    while (w.i >= 0 && w.j >= 0) {
-        if (walk_step<GAP>(w, lane) == 0) break;
-        if (lane == (cnt & (WAVE - 1))) {
-            pi = w.i;
-            pj = w.j;
+    // path cells cnt .. cnt + n - 1: (i0 - di (t + 1), j0 - dj (t + 1)), t < n
+    auto record = [&](int n, int i0, int j0, int di, int dj) {
+        int t0 = 0;
+        while (t0 < n) {
+            const int slot = cnt & (WAVE - 1);
+            const int take = n - t0 < WAVE - slot ? n - t0 : WAVE - slot;
+            const int t = t0 + (lane - slot) + 1;
+            if (lane >= slot && lane < slot + take) {
+                pi = i0 - di * t;
+                pj = j0 - dj * t;
+            }
+            cnt += take;
+            t0 += take;
+            if ((cnt & (WAVE - 1)) == 0) spill(cnt);
         }
-        if ((++cnt & (WAVE - 1)) == 0) spill(cnt);
+    };
+    // The moves of walk_step a run at a time (as the dense backward takes
+    // them): lane m reads the cell m moves on along the run the path is in
+    // (cells of one code; or an affine gap run, whose cells extend it while
+    // their extend bit is set), and a ballot gives the run's length.
+    int mode = 0;  // affine: 2 / 3 inside a vertical / horizontal gap run
+    while (w.i >= 0 && w.j >= 0) {
+        const uint32_t c0 = w.code(lane);  // (loads a tile holding the current cell)
+        if (mode == 0) {
+            const int src = int(c0 & 3u);
+            if (src == 0) break;  // a local start
+            if constexpr (GAP == GAP_AFF) {
+                if (src >= 2) {  // a gap run starts at this cell
+                    mode = src;
+                    continue;
+                }
+            }
+            const int di = src != 3 ? 1 : 0, dj = src != 2 ? 1 : 0;
+            bool ok;
+            const uint32_t cm = w.gather(di, dj, lane, ok);
+            const uint64_t run = __ballot(ok && int(cm & 3u) == src);
+            const int n = ~run == 0 ? WAVE : __builtin_ctzll(~run);  // (at least 1: the current cell)
+            record(n, w.i, w.j, di, dj);
+            w.jump(w.i - di * n, w.j - dj * n);
+        } else {
+            const bool vert = mode == 2;
+            bool ok;
+            const uint32_t cm = w.gather(vert ? 1 : 0, vert ? 0 : 1, lane, ok);
+            const int pos = (vert ? w.i : w.j) - lane;
+            const uint64_t ext = __ballot(ok && pos > 0 && (cm & (vert ? 4u : 8u)) != 0u);
+            const int n = ~ext == 0 ? WAVE : __builtin_ctzll(~ext);
+            const uint64_t valid = __ballot(ok);
+            if (n < WAVE && ((valid >> n) & 1u)) {  // the run's last cell: one more move ends it
+                w.jump(w.i - (vert ? n + 1 : 0), w.j - (vert ? 0 : n + 1));
+                record(1, w.i + (vert ? 1 : 0), w.j + (vert ? 0 : 1), vert ? 1 : 0, vert ? 0 : 1);
+                mode = 0;
+            } else {  // the run goes on past the tile
+                w.jump(w.i - (vert ? n : 0), w.j - (vert ? 0 : n));
+            }
+        }
     }
     if (cnt & (WAVE - 1)) spill(cnt);
     if (lane == 0) path_len[out] = cnt;

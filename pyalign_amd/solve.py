@@ -11,8 +11,6 @@ SolverCache does.
 """
 
 import contextlib
-import gc
-import threading
 import itertools
 import time
 import typing
@@ -433,36 +431,9 @@ def _encode_bag(problems, which):
     return flat, off, lens
 
 
-@contextlib.contextmanager
-def _collector_paused():
-    """Python's cyclic garbage collector paused while a bag's result objects
-    are built: thousands of new Solutions otherwise trigger collections that
-    traverse every live object of the process (4,096 Solutions of 100 x 100
-    global alignments: 13 ms per solve paused, 21 ms median and up to 110 ms
-    beside 2 M live objects without).  Reference counting still frees
-    acyclic garbage at once; cycles wait for the next collection.
-
-    The collector's switch is process-wide: builds in several threads share
-    one pause (a count under a lock), which ends with the last of them, and
-    a collector that was off at the first of them stays off."""
-    global _pause_count, _pause_restore
-    with _pause_lock:
-        if _pause_count == 0:
-            _pause_restore = gc.isenabled()
-            gc.disable()
-        _pause_count += 1
-    try:
-        yield
-    finally:
-        with _pause_lock:
-            _pause_count -= 1
-            if _pause_count == 0 and _pause_restore:
-                gc.enable()
-
-
-_pause_lock = threading.Lock()
-_pause_count = 0
-_pause_restore = False
+# Python's cyclic collector paused while result objects are built (shared
+# with the library layer, which builds the core results)
+_collector_paused = algorithm.collector_paused
 
 
 def _joined_text(problems):
