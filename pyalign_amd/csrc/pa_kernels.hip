@@ -327,152 +327,6 @@ __device__ __forceinline__ int walk_step(TbWalker &w, int lane) {
     return src;
 }
 
-// The walk of a problem laid out in strips of 64 rows, one row per lane (a
-// step's codes are one 64-word row, a strip's steps consecutive rows): the
-// same moves as walk_step over a tile of whole rows, the current cell's
-// word at tile index (s - s0) * 64 + k, so a move is a change of that
-// index (up -65, left -64) and a tile is one contiguous block, loaded four
-// words per lane and load.  Returns the path's length; cells are stored as
-// pa_walk_kernel stores them.
-template <int GAP>
-__device__ __forceinline__ int walk_rows64(const Prob &pr, const uint32_t *tb, uint32_t *lds, int tile_words, int u, int v,
-                           int32_t *P, int lane) {
-    constexpr uint32_t MASK = GAP == GAP_AFF ? 15u : 3u;
-    const int nsteps = pr.len_t + WAVE - 1;
-    // tile rows (steps): a diagonal path crosses a strip's 64 lanes in 128
-    // steps; the whole tile is one round of loads (32 per lane)
-    const int ST = tile_words / WAVE < 128 ? tile_words / WAVE : 128;
-    int i = u, j = v;
-    int st = i / WAVE, k = i - st * WAVE, s = j + k;
-    int t_st = -1, s0 = 0, idx = 0;
-    const bool vec = (pr.tb_off & 3) == 0;
-    auto load = [&]() {
-        t_st = st;
-        const int s1 = s + 1;
-        s0 = s1 > ST ? s1 - ST : 0;
-        idx = (s - s0) * WAVE + k;
-        const uint32_t *base = tb + pr.tb_off + (int64_t(t_st) * nsteps + s0) * WAVE;
-        const int cnt = (s1 - s0) * WAVE;  // words, a multiple of 64
-        __builtin_amdgcn_wave_barrier();   // the old tile's reads are done
-        if (vec) {
-            // loads straight into LDS (16 bytes a lane, lane l at the
-            // instruction's base + 16 l): the whole tile in one round, no
-            // registers held
-            const uint4 *b4 = reinterpret_cast<const uint4 *>(base);
-            const int n4 = cnt / 4;
-            for (int q = 0; q * WAVE < n4; ++q) {
-                if (q * WAVE + lane < n4)
-                    __builtin_amdgcn_global_load_lds(
-                        (const __attribute__((address_space(1))) void *)(b4 + q * WAVE + lane),
-                        (__attribute__((address_space(3))) void *)(lds + q * WAVE * 4), 16, 0, 0);
-            }
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        } else {
-            constexpr int U = 32;
-            for (int c0 = 0; c0 < cnt; c0 += U * WAVE) {
-                uint32_t x[U];
-#pragma unroll
-                for (int q = 0; q < U; ++q) {
-                    const int c = c0 + q * WAVE + lane;
-                    x[q] = c < cnt ? base[c] : 0u;
-                }
-#pragma unroll
-                for (int q = 0; q < U; ++q) {
-                    const int c = c0 + q * WAVE + lane;
-                    if (c < cnt) lds[c] = x[q];
-                }
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-    // A window of the cells the path can reach in its next few moves, one
-    // word per lane: lane 6 a + b holds the cell a left or up moves and b
-    // up moves on (tile index idx - 64 a - b, a < 10, b < 6), read once;
-    // a move then takes its code from a register of that lane (readlane)
-    // instead of an LDS round trip.  (wa, wb) is the current cell's place in
-    // the window; leaving it, or a new tile, reads a new window.
-    constexpr int WB = 6, WA = 10;
-    uint32_t win = 0;
-    int wa = WA, wb = 0;
-    auto refresh = [&]() {
-        wa = wb = 0;
-        const int q = lane < WA * WB ? lane : 0;
-        const int a = q / WB, b = q - a * WB;
-        const int x = idx - WAVE * a - b;
-        win = lds[x > 0 ? x : 0];
-    };
-    auto code = [&]() -> uint32_t {
-        if (st != t_st || s < s0) {
-            load();
-            refresh();
-        } else if (wa >= WA || wb >= WB) {
-            refresh();
-        }
-        return uint32_t(__builtin_amdgcn_readlane(int(win), wa * WB + wb)) & MASK;
-    };
-    auto up = [&]() {
-        --i;
-        --s;
-        idx -= WAVE + 1;
-        ++wa;
-        ++wb;
-        if (--k < 0) {  // into the strip above (its tile is loaded on the next read)
-            k = WAVE - 1;
-            s += WAVE;
-            --st;
-        }
-    };
-    auto left = [&]() {
-        --j;
-        --s;
-        idx -= WAVE;
-        ++wa;
-    };
-    int pi = u, pj = v;  // cell 0 sits in lane 0
-    int cnt = 1;
-    auto spill = [&](int upto) {  // cells [upto & ~63, upto)
-        const int base = (upto - 1) & ~(WAVE - 1);
-        if (base + lane < upto) {
-            P[2 * (base + lane)] = pi;
-            P[2 * (base + lane) + 1] = pj;
-        }
-    };
-    while (i >= 0 && j >= 0) {
-        uint32_t c = code();
-        const uint32_t src = c & 3u;
-        if (src == 0) break;
-        if (src == 1) {
-            up();
-            left();
-        } else if (src == 2) {
-            if constexpr (GAP == GAP_AFF) {
-                while (i > 0 && (c & 4u)) {  // (the first test reads the code already at hand)
-                    up();
-                    c = code();
-                }
-            }
-            up();
-        } else {
-            if constexpr (GAP == GAP_AFF) {
-                while (j > 0 && (c & 8u)) {
-                    left();
-                    c = code();
-                }
-            }
-            left();
-        }
-        if (lane == (cnt & (WAVE - 1))) {
-            pi = i;
-            pj = j;
-        }
-        if ((++cnt & (WAVE - 1)) == 0) spill(cnt);
-    }
-    if (cnt & (WAVE - 1)) spill(cnt);
-    return cnt;
-}
-
 // The walk of a long-mode problem (strips of 64 rows, one row per lane,
 // traceback codes packed along the steps: long_pack(bits) steps per word,
 // see long_strip_words): the moves of walk_step, taken a run at a time.
@@ -665,11 +519,6 @@ __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t
     int32_t *P = path + 2 * path_off[out];
     if (pr.pack) {
         const int cnt = walk_long64<GAP>(pr, tb, smem + wave * tile_words, tile_words, u, v, P, lane);
-        if (lane == 0) path_len[out] = cnt;
-        return;
-    }
-    if (pr.L == WAVE && pr.R == 1) {
-        const int cnt = walk_rows64<GAP>(pr, tb, smem + wave * tile_words, tile_words, u, v, P, lane);
         if (lane == 0) path_len[out] = cnt;
         return;
     }
