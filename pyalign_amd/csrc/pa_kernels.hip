@@ -801,6 +801,84 @@ __global__ __launch_bounds__(256) void pa_backward_kernel(BwArgs a, int tile_wor
                     }
                 }
             }
+        } else if constexpr (FORM == F_SERIES && GAP == GAP_DTW) {
+            // DTW over series, a run at a time (moves of one direction):
+            // lane t computes the cost gradient of the run's cell t.  A row
+            // (column) the path crosses is one contiguous stretch of its
+            // cells, summed in path order: across a run (diagonal, or up for
+            // rows) each cell's row is complete within the run and its sum
+            // (the pending one, for the run's first cell) goes out at once;
+            // along a run (left for rows, up for columns) the cells add to
+            // the running sum in order, lane by lane.  The sums are those of
+            // cost_grad, bit for bit.
+            while (w.i >= 0 && w.j >= 0) {
+                const uint32_t c0 = w.code(lane);  // (loads a tile holding the current cell)
+                wx.row(w.i, lane);                 // (series windows holding the current cell)
+                wy.row(w.j, lane);
+                const int src = int(c0 & 3u);
+                if (src == 0) {
+                    cost_grad(w.i, w.j, w.ga);
+                    break;
+                }
+                const int di = src != 3 ? 1 : 0, dj = src != 2 ? 1 : 0;
+                bool ok;
+                const uint32_t cm = w.gather(di, dj, lane, ok);
+                const int ii = w.i - di * lane, jj = w.j - dj * lane;
+                ok = ok && ii >= wx.lo && jj >= wy.lo;
+                const uint64_t run = __ballot(ok && int(cm & 3u) == src);
+                const int n = ~run == 0 ? WAVE : __builtin_ctzll(~run);  // (at least 1: the current cell)
+                float gd[SDD];
+                const bool mine = lane < n;
+#pragma unroll
+                for (int d = 0; d < SD; ++d) {
+                    const float xv = mine ? wx.lds[(ii - wx.lo) * SD + d] : 0.0f;
+                    const float yv = mine ? wy.lds[(jj - wy.lo) * SD + d] : 0.0f;
+                    gd[d] = g * (2.0f * (xv - yv));
+                }
+                // rows: complete per cell across the run, a running sum along it
+                if (di == 1) {
+                    if (mine && a.grad_xs) {
+                        const bool cont = lane == 0 && cur_i == w.i;
+#pragma unroll
+                        for (int d = 0; d < SD; ++d)
+                            add_noret(a.grad_xs + (pr.off_s + ii) * SD + d, (cont ? gx[d] : 0.0f) + gd[d]);
+                    }
+                    cur_i = -1;  // (nothing pending)
+                } else {
+                    if (cur_i != w.i) {
+                        flush_row();
+                        cur_i = w.i;
+#pragma unroll
+                        for (int d = 0; d < SD; ++d) gx[d] = 0.0f;
+                    }
+                    for (int t = 0; t < n; ++t)
+#pragma unroll
+                        for (int d = 0; d < SD; ++d)
+                            gx[d] += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gd[d]), t));
+                }
+                // columns likewise
+                if (dj == 1) {
+                    if (mine && a.grad_xt) {
+                        const bool cont = lane == 0 && cur_j == w.j;
+#pragma unroll
+                        for (int d = 0; d < SD; ++d)
+                            add_noret(a.grad_xt + (pr.off_t + jj) * SD + d, (cont ? gy[d] : 0.0f) - gd[d]);
+                    }
+                    cur_j = -1;
+                } else {
+                    if (cur_j != w.j) {
+                        flush_col();
+                        cur_j = w.j;
+#pragma unroll
+                        for (int d = 0; d < SD; ++d) gy[d] = 0.0f;
+                    }
+                    for (int t = 0; t < n; ++t)
+#pragma unroll
+                        for (int d = 0; d < SD; ++d)
+                            gy[d] -= __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gd[d]), t));
+                }
+                w.jump(w.i - di * n, w.j - dj * n);
+            }
         } else {
             while (w.i >= 0 && w.j >= 0) {
                 const int pu = w.i, pv = w.j;
