@@ -409,7 +409,7 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
         const int q = uu >> LGPK;
         ok = r >= 0 && q >= q0;
         const uint32_t w = ok ? lds[cur + (q - q0) * WAVE + r] : 0u;
-        return (w >> ((TB * uu) & 31)) & MASK;
+        return (w >> (TB * (uu & (PK - 1)))) & MASK;  // (PK may be below 32 / TB: 8-step phases)
     };
     // a run of n moves of di rows and dj columns (the last one may leave the strip)
     auto advance = [&](int n, int di, int dj) {
