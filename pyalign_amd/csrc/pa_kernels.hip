@@ -327,6 +327,50 @@ __device__ __forceinline__ int walk_step(TbWalker &w, int lane) {
     return src;
 }
 
+// A walk's path cells, held one per lane (cell c in lane c % 64) and stored
+// 64 at a time with one coalesced store, so that no move waits on a store of
+// its own (the wait counter is shared with the tile loads).  Cell 0, the
+// seed, sits in lane 0.
+struct PathCells {
+    int32_t *P;
+    int pi, pj;  // this lane's cell
+    int cnt;     // cells so far
+
+    __device__ void init(int32_t *p, int u, int v) {
+        P = p;
+        pi = u;
+        pj = v;
+        cnt = 1;
+    }
+    __device__ void spill(int upto, int lane) {  // cells [upto & ~63, upto)
+        const int b0 = (upto - 1) & ~(WAVE - 1);
+        if (b0 + lane < upto) {
+            P[2 * (b0 + lane)] = pi;
+            P[2 * (b0 + lane) + 1] = pj;
+        }
+    }
+    // cells cnt .. cnt + n - 1: (i0 - di (t + 1), j0 - dj (t + 1)), t < n
+    __device__ void record(int n, int i0, int j0, int di, int dj, int lane) {
+        int t0 = 0;
+        while (t0 < n) {
+            const int slot = cnt & (WAVE - 1);
+            const int take = n - t0 < WAVE - slot ? n - t0 : WAVE - slot;
+            const int t = t0 + (lane - slot) + 1;
+            if (lane >= slot && lane < slot + take) {
+                pi = i0 - di * t;
+                pj = j0 - dj * t;
+            }
+            cnt += take;
+            t0 += take;
+            if ((cnt & (WAVE - 1)) == 0) spill(cnt, lane);
+        }
+    }
+    __device__ int finish(int lane) {  // stores the last cells, returns the path's length
+        if (cnt & (WAVE - 1)) spill(cnt, lane);
+        return cnt;
+    }
+};
+
 // The walk of a long-mode problem (strips of 64 rows, one row per lane,
 // traceback codes packed along the steps: long_pack(bits) steps per word,
 // see long_strip_words): the moves of walk_step, taken a run at a time.
@@ -376,31 +420,8 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
         }
         return qa;
     };
-    int pi = u, pj = v;  // cell 0 sits in lane 0
-    int cnt = 1;
-    auto spill = [&](int upto) {  // cells [upto & ~63, upto)
-        const int b0 = (upto - 1) & ~(WAVE - 1);
-        if (b0 + lane < upto) {
-            P[2 * (b0 + lane)] = pi;
-            P[2 * (b0 + lane) + 1] = pj;
-        }
-    };
-    // path cells cnt .. cnt + n - 1: (i0 - di (t + 1), j0 - dj (t + 1)), t < n
-    auto record = [&](int n, int i0, int j0, int di, int dj) {
-        int t0 = 0;
-        while (t0 < n) {
-            const int slot = cnt & (WAVE - 1);
-            const int take = n - t0 < WAVE - slot ? n - t0 : WAVE - slot;
-            const int t = t0 + (lane - slot) + 1;
-            if (lane >= slot && lane < slot + take) {
-                pi = i0 - di * t;
-                pj = j0 - dj * t;
-            }
-            cnt += take;
-            t0 += take;
-            if ((cnt & (WAVE - 1)) == 0) spill(cnt);
-        }
-    };
+    PathCells cells;
+    cells.init(P, u, v);
     // the code of the cell dk * lane rows up and du * lane steps back (for
     // this lane), and whether that cell lies in the strip and the tile
     auto gather = [&](int dk, int du, bool &ok) -> uint32_t {
@@ -458,7 +479,7 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
             const int n = ~run == 0 ? WAVE : __builtin_ctzll(~run);
             const uint64_t valid = __ballot(ok);
             if (n > 0) {
-                record(n, i, j, 1, 1);
+                cells.record(n, i, j, 1, 1, lane);
                 advance(n, 1, 1);
             }
             // the cell the run stopped at, when it lies in this strip and tile
@@ -469,7 +490,7 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
                 if constexpr (GAP == GAP_AFF) {
                     mode = int(src);  // the gap run starts at this cell
                 } else {
-                    record(1, i, j, src == 2 ? 1 : 0, src == 3 ? 1 : 0);
+                    cells.record(1, i, j, src == 2 ? 1 : 0, src == 3 ? 1 : 0, lane);
                     advance(1, src == 2 ? 1 : 0, src == 3 ? 1 : 0);
                 }
             }
@@ -485,7 +506,7 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
             const uint64_t valid = __ballot(ok);
             if (n < WAVE && ((valid >> n) & 1u)) {  // the run's last cell: one more move ends it
                 const int di = vert ? 1 : 0, dj = vert ? 0 : 1;
-                record(1, i - di * n, j - dj * n, di, dj);
+                cells.record(1, i - di * n, j - dj * n, di, dj, lane);
                 advance(n + 1, di, dj);
                 mode = 0;
             } else {  // the run goes on past this strip or tile
@@ -493,8 +514,7 @@ __device__ __forceinline__ int walk_long64(const Prob &pr, const uint32_t *tb, u
             }
         }
     }
-    if (cnt & (WAVE - 1)) spill(cnt);
-    return cnt;
+    return cells.finish(lane);
 }
 
 template <int GAP>
@@ -525,34 +545,8 @@ __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t
     TbWalker w;
     w.init(smem + wave * tile_words, tile_words, tb, pr, GAP == GAP_AFF ? 4 : 2);
     w.seek(u, v);
-    // path cells are held one per lane (cell cnt in lane cnt % 64) and
-    // stored 64 at a time with one coalesced store, so no move waits on a
-    // store of its own (the wait counter is shared with the tile loads)
-    int pi = u, pj = v;  // cell 0 sits in lane 0
-    int cnt = 1;
-    auto spill = [&](int upto) {  // cells [upto & ~63, upto)
-        const int base = (upto - 1) & ~(WAVE - 1);
-        if (base + lane < upto) {
-            P[2 * (base + lane)] = pi;
-            P[2 * (base + lane) + 1] = pj;
-        }
-    };
-    // path cells cnt .. cnt + n - 1: (i0 - di (t + 1), j0 - dj (t + 1)), t < n
-    auto record = [&](int n, int i0, int j0, int di, int dj) {
-        int t0 = 0;
-        while (t0 < n) {
-            const int slot = cnt & (WAVE - 1);
-            const int take = n - t0 < WAVE - slot ? n - t0 : WAVE - slot;
-            const int t = t0 + (lane - slot) + 1;
-            if (lane >= slot && lane < slot + take) {
-                pi = i0 - di * t;
-                pj = j0 - dj * t;
-            }
-            cnt += take;
-            t0 += take;
-            if ((cnt & (WAVE - 1)) == 0) spill(cnt);
-        }
-    };
+    PathCells cells;
+    cells.init(P, u, v);
     // The moves of walk_step a run at a time (as the dense backward takes
     // them): lane m reads the cell m moves on along the run the path is in
     // (cells of one code; or an affine gap run, whose cells extend it while
@@ -574,7 +568,7 @@ __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t
             const uint32_t cm = w.gather(di, dj, lane, ok);
             const uint64_t run = __ballot(ok && int(cm & 3u) == src);
             const int n = ~run == 0 ? WAVE : __builtin_ctzll(~run);  // (at least 1: the current cell)
-            record(n, w.i, w.j, di, dj);
+            cells.record(n, w.i, w.j, di, dj, lane);
             w.jump(w.i - di * n, w.j - dj * n);
         } else {
             const bool vert = mode == 2;
@@ -586,14 +580,14 @@ __global__ __launch_bounds__(256) void pa_walk_kernel(const Prob *probs, int64_t
             const uint64_t valid = __ballot(ok);
             if (n < WAVE && ((valid >> n) & 1u)) {  // the run's last cell: one more move ends it
                 w.jump(w.i - (vert ? n + 1 : 0), w.j - (vert ? 0 : n + 1));
-                record(1, w.i + (vert ? 1 : 0), w.j + (vert ? 0 : 1), vert ? 1 : 0, vert ? 0 : 1);
+                cells.record(1, w.i + (vert ? 1 : 0), w.j + (vert ? 0 : 1), vert ? 1 : 0, vert ? 0 : 1, lane);
                 mode = 0;
             } else {  // the run goes on past the tile
                 w.jump(w.i - (vert ? n : 0), w.j - (vert ? 0 : n));
             }
         }
     }
-    if (cnt & (WAVE - 1)) spill(cnt);
+    const int cnt = cells.finish(lane);
     if (lane == 0) path_len[out] = cnt;
 }
 
