@@ -318,7 +318,9 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         // its diagonal, so one bank would cost a register copy per cell)
         auto step = [&](auto phase, auto keepv, int s, float (&Din)[R], float (&Dout)[R], Col &cur) {
             constexpr int C = decltype(phase)::value;  // s % 8 (staged dense rows only)
-            constexpr bool KEEPV = decltype(keepv)::value;  // cell values stored (a.dvals set)
+            constexpr bool KEEPV = decltype(keepv)::value == 1;  // cell values stored (a.dvals set)
+            // no traceback stored (a checkpointed forward's group): the codes are not computed
+            constexpr bool NOTB = decltype(keepv)::value == 2;
             const int j = s - l;
             const int tsym = cur.sym;
             [[maybe_unused]] float tv[SDD];
@@ -482,7 +484,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                             }
                         }
                     }
-                    if constexpr (TR != TR_SCORE) tbw[(r * BITS) / 32] |= bits << ((r * BITS) % 32);
+                    if constexpr (TR != TR_SCORE && !NOTB) tbw[(r * BITS) / 32] |= bits << ((r * BITS) % 32);
 
                     Dd = Din[r];
                     Dout[r] = D;
@@ -501,7 +503,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                     }
                 }
 
-                if constexpr (TR != TR_SCORE) {
+                if constexpr (TR != TR_SCORE && !NOTB) {
                     // problems of a checkpointed forward keep no traceback
                     if (pr.tb_off != NO_TB_OFF) {
                         pend_dst = a.tb + pr.tb_off + (int64_t(st) * nsteps_p + s) * int64_t(L) * WPC + l * WPC;
@@ -530,8 +532,10 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
             }
         };
         using std::integral_constant;
-        // the step loop, built twice where cell values may be kept so that
-        // the usual loop carries no per-cell test of a.dvals
+        // the step loop, built once per kind of group so that the usual loop
+        // carries no per-cell test: 0 codes stored, 1 codes and cell values
+        // (a.dvals), 2 neither (the groups of a checkpointed forward keep
+        // strip rows only; their folds keep their order, their codes go)
         auto sweep = [&](auto kv) {
             int s = 0;
             if constexpr (STAGED) {
@@ -554,11 +558,15 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 if (s < steps) step(integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
             }
         };
-        if constexpr (TR != TR_SCORE) {
-            if (a.dvals != nullptr) sweep(std::true_type{});
-            else sweep(std::false_type{});
+        if constexpr (TR == TR_ONE) {
+            if (a.dvals != nullptr) sweep(integral_constant<int, 1>{});
+            else if (a.probs[g.first].tb_off == NO_TB_OFF) sweep(integral_constant<int, 2>{});  // (one problem a group)
+            else sweep(integral_constant<int, 0>{});
+        } else if constexpr (TR != TR_SCORE) {
+            if (a.dvals != nullptr) sweep(integral_constant<int, 1>{});
+            else sweep(integral_constant<int, 0>{});
         } else {
-            sweep(std::false_type{});
+            sweep(integral_constant<int, 0>{});
         }
         if constexpr (LOC == LOC_GLOBAL) {
             // D(ls-1, lt-1): column lt-1 was this lane's step lt-1+l, which
