@@ -137,18 +137,24 @@ class ProblemBag:
 
     def __init__(self, problems):
         self._problems = problems
+        classes = set(map(type, problems))
+        self._classes = classes
+        # where no class overrides the properties, the stored values are read
+        # directly (no property call per problem)
+        plain = all(getattr(c, name, None) is getattr(Problem, name) for c in classes
+                    for name in ("direction", "dtype", "shape"))
+        get = (lambda name: attrgetter("_" + name)) if plain else attrgetter
         # list.count compares in C (identity first): every item equal to the first
         self._direction = problems[0].direction
-        if list(map(attrgetter("direction"), problems)).count(self._direction) != len(problems):
+        if list(map(get("direction"), problems)).count(self._direction) != len(problems):
             raise ValueError("problems in a bag need to have same direction")
         self._dtype = problems[0].dtype
-        if list(map(attrgetter("dtype"), problems)).count(self._dtype) != len(problems):
+        if list(map(get("dtype"), problems)).count(self._dtype) != len(problems):
             raise ValueError("problems in a bag need to have same dtype")
         # a problem's form is fixed by its class: one lookup per class
-        classes = set(map(type, problems))
         reps = [problems[0]] if len(classes) == 1 else [next(p for p in problems if type(p) is c) for c in classes]
         self._form = Form(min(p.form.value for p in reps))
-        shapes = np.fromiter(chain.from_iterable(map(attrgetter("shape"), problems)), dtype=np.int64,
+        shapes = np.fromiter(chain.from_iterable(map(get("shape"), problems)), dtype=np.int64,
                              count=2 * len(problems)).reshape(len(problems), 2)
         self._shapes = shapes
         self._max_shape = (np.max(shapes[:, 0]), np.max(shapes[:, 1]))
@@ -180,6 +186,11 @@ class ProblemBag:
     @property
     def form(self):
         return self._form
+
+    @property
+    def classes(self):
+        """The set of the problems' classes."""
+        return self._classes
 
     def batches(self, batch_size):
         """Greedy batches: the largest remaining Morton code is the master

@@ -23,7 +23,8 @@ from operator import attrgetter, is_, methodcaller
 from . import algorithm
 from .gaps import GapCost, ConstantGapCost
 from .problems import ProblemBag, Form
-from .problems.factory import AlphabetEncoder, SeriesProblem
+from .problems.instance import Problem
+from .problems.factory import AlphabetEncoder, AlphabetProblem, SeriesProblem
 
 
 class Solution:
@@ -386,8 +387,13 @@ def _all_are(values, obj):
     return all(map(is_, values, itertools.repeat(obj)))
 
 
-def _all_have(problems, name, obj):
-    """Whether every problem's attribute name is obj; a missing attribute is False."""
+def _all_have(problems, name, obj, classes=()):
+    """Whether every problem's attribute name is obj; a missing attribute is
+    False.  classes: the problems' classes; where all of them inherit the
+    alphabet problems' encoder property, its stored value is read directly."""
+    if name == "encoder" and classes and all(getattr(c, "encoder", None) is AlphabetProblem.encoder
+                                             for c in classes):
+        name = "_encoder"
     try:
         return _all_are(map(attrgetter(name), problems), obj)
     except AttributeError:
@@ -396,7 +402,13 @@ def _all_have(problems, name, obj):
 
 def _shared_table(bag):
     sim = bag.problems[0].similarity_lookup_table()
-    if not _all_are(map(methodcaller("similarity_lookup_table"), bag.problems), sim):
+    # (alphabet problems return their stored table: read it without a call each)
+    if all(getattr(c, "similarity_lookup_table", None) is AlphabetProblem.similarity_lookup_table
+           for c in bag.classes):
+        same = _all_are(map(attrgetter("_matrix"), bag.problems), sim)
+    else:
+        same = _all_are(map(methodcaller("similarity_lookup_table"), bag.problems), sim)
+    if not same:
         raise ValueError("similarity table must be identical for all problems in a batch")
     return sim
 
@@ -436,13 +448,18 @@ def _encode_bag(problems, which):
 _collector_paused = algorithm.collector_paused
 
 
-def _joined_text(problems):
+def _joined_text(problems, classes=()):
     """The bag's s and t texts joined (latin-1 bytes when every symbol fits,
     else UTF-32-LE code points) for encoding on the device, or None when a
-    sequence is not a str."""
+    sequence is not a str.  classes: the problems' classes; where none of
+    them overrides the s and t properties the stored sequences are read
+    directly (no property call per problem)."""
+    plain = bool(classes) and all(getattr(c, "s", None) is Problem.s and getattr(c, "t", None) is Problem.t
+                                  for c in classes)
+    get_s, get_t = (attrgetter("_s"), attrgetter("_t")) if plain else (attrgetter("s"), attrgetter("t"))
     try:  # join accepts only str items: lists or arrays of symbols go to the host encoders
-        js = "".join(map(attrgetter("s"), problems))
-        jt = "".join(map(attrgetter("t"), problems))
+        js = "".join(map(get_s, problems))
+        jt = "".join(map(get_t, problems))
     except TypeError:
         return None
     try:
@@ -451,15 +468,15 @@ def _joined_text(problems):
         return js, jt, js.encode("utf-32-le"), jt.encode("utf-32-le"), 4
 
 
-def _text_payload(problems, ls, lt, indexed):
+def _text_payload(problems, ls, lt, indexed, classes=()):
     """Raw-symbol payload of a bag of str problems (the device encodes it):
     indexed problems need their shared encoder's symbol table (char_lut)."""
     lut = None
     enc = getattr(problems[0], "encoder", None)
     if indexed:
-        if not hasattr(enc, "char_lut") or not _all_have(problems, "encoder", enc):
+        if not hasattr(enc, "char_lut") or not _all_have(problems, "encoder", enc, classes):
             return None, None
-    text = _joined_text(problems)
+    text = _joined_text(problems, classes)
     if text is None:
         return None, None
     js, jt, rs, rt, width = text
@@ -562,7 +579,8 @@ class Solver:
             lt = bag.shapes[:, 1].astype(np.int32)
             text = None
             if bag.form in (Form.INDEXED_MATRIX_FORM, Form.BINARY_MATRIX_FORM) and problems:
-                payload, text = _text_payload(problems, ls, lt, bag.form == Form.INDEXED_MATRIX_FORM)
+                payload, text = _text_payload(problems, ls, lt, bag.form == Form.INDEXED_MATRIX_FORM,
+                                              bag.classes)
             if text is not None:
                 # str sequences: the device encodes the joined text
                 if bag.form == Form.INDEXED_MATRIX_FORM:
