@@ -188,6 +188,7 @@ struct pa_solver {
     // raw symbols encoded on the device (pa_ragged.raw_width)
     DevBuf b_raw_s, b_raw_t, b_lut, b_bad;
     DevBuf b_safe, b_sink;  // KArgs::safe, KArgs::sink
+    DevBuf b_finite;        // KArgs::finite (series form)
     DevBuf b_ckpt_tb;       // checkpointed backward: recomputed tracebacks (scratch)
     DevBuf b_tape_borders;  // a checkpointed tape's copy of the border vectors (recycled)
     // pinned staging of border uploads after a gap change (pa_solver_set_gap_params)
@@ -1327,6 +1328,17 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     a.border_t = s.b_border_t.as<float>();
     a.safe = s.b_safe.as<float>();
     a.sink = s.b_sink.as<uint32_t>();
+    // DTW over series: a check of the series' values before the DP launches
+    // lets the kernels fold with min / max where every value is finite
+    int64_t finite_nx = 0, finite_ny = 0;
+    if (job.form == F_SERIES && job.gap == GAP_DTW && in.off_s && in.off_t) {
+        s.b_finite.ensure(4);
+        a.finite = s.b_finite.as<uint32_t>();
+        for (int64_t k = 0; k < n; k++) {
+            if (in.len_s[k] > 0) finite_nx = std::max<int64_t>(finite_nx, (in.off_s[k] + in.len_s[k]) * in.dim);
+            if (in.len_t[k] > 0) finite_ny = std::max<int64_t>(finite_ny, (in.off_t[k] + in.len_t[k]) * in.dim);
+        }
+    }
     a.go_s = s.go_s;
     a.ge_s = s.ge_s;
     a.go_t = s.go_t;
@@ -1387,6 +1399,9 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
     // launch (the critical path), the DP launches, the band fold
     auto enqueue = [&]() -> hipError_t {
         hipError_t e = launch_zero_u32(s.b_queue.as<uint32_t>(), int64_t(plan.launches.size()) + 1, st);
+        if (e == hipSuccess && a.finite)
+            e = launch_all_finite(static_cast<const float *>(codes_s), finite_nx, static_cast<const float *>(codes_t),
+                                  finite_ny, s.b_finite.as<uint32_t>(), st);
         if (e == hipSuccess && plan.band_slots > 0) e = launch_zero_u32(s.b_progress.as<uint32_t>(), plan.band_slots, st);
         if (e == hipSuccess && plan.band_slots > 0) e = launch_zero_u32(s.b_band_err.as<uint32_t>(), 1, st);
         if (e == hipSuccess && plan.xbuf_end > plan.xbuf_begin)
@@ -1495,12 +1510,17 @@ void run(pa_solver &s, const Input &in, std::unique_ptr<pa_result> *result, floa
             const size_t ns = s.border_s.size(), nt = s.border_t.size();
             std::swap(tape->borders.p, s.b_tape_borders.p);
             std::swap(tape->borders.cap, s.b_tape_borders.cap);
-            tape->borders.ensure((ns + nt) * 4);
+            tape->borders.ensure((ns + nt + 1) * 4);
             HIPCHK(hipMemcpyAsync(tape->borders.p, s.b_border_s.p, ns * 4, hipMemcpyDeviceToDevice, st));
             HIPCHK(hipMemcpyAsync(tape->borders.as<float>() + ns, s.b_border_t.p, nt * 4, hipMemcpyDeviceToDevice,
                                   st));
             tape->kargs.border_s = tape->borders.as<float>();
             tape->kargs.border_t = tape->borders.as<float>() + ns;
+            if (a.finite) {  // and of this forward's series check (after the borders)
+                uint32_t *f = reinterpret_cast<uint32_t *>(tape->borders.as<float>() + ns + nt);
+                HIPCHK(hipMemcpyAsync(f, a.finite, 4, hipMemcpyDeviceToDevice, st));
+                tape->kargs.finite = f;
+            }
         }
         s.last_plan.uploaded = false;  // its descriptors moved to the tape
         return;

@@ -164,6 +164,10 @@ struct KArgs {
     const float *safe;
     // SINK_WORDS writable words: where a lane with nothing to store stores
     uint32_t *sink;
+    // series form (DTW): a word that holds 1 when every series value is
+    // finite, so that the cells' costs and values are never NaN or -0 and
+    // min / max folds return the strict folds' bits (null: strict folds)
+    const uint32_t *finite;
 };
 constexpr int SAFE_FLOATS = 16;
 constexpr int SINK_WORDS = 16;
@@ -241,6 +245,9 @@ struct StripWalk {
 hipError_t launch_backward_strips(int gap, int form, int dim, const BwArgs &args, StripWalk *state,
                                   const int32_t *strip_of, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t value, hipStream_t stream);
+// *flag = 1 when the nx floats at x and the ny floats at y are all finite, else 0
+hipError_t launch_all_finite(const float *x, int64_t nx, const float *y, int64_t ny, uint32_t *flag,
+                             hipStream_t stream);
 // Raw symbols (width 1 or 4 bytes) to codes: uint8 or uint16 (out_width 1,
 // 2) through lut, or uint32 values (out_width 4) when lut is null; *bad gets
 // the smallest position without a code.

@@ -318,9 +318,12 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         // its diagonal, so one bank would cost a register copy per cell)
         auto step = [&](auto phase, auto keepv, int s, float (&Din)[R], float (&Dout)[R], Col &cur) {
             constexpr int C = decltype(phase)::value;  // s % 8 (staged dense rows only)
-            constexpr bool KEEPV = decltype(keepv)::value == 1;  // cell values stored (a.dvals set)
+            constexpr int KM = decltype(keepv)::value;
+            constexpr bool KEEPV = KM == 1;  // cell values stored (a.dvals set)
             // no traceback stored (a checkpointed forward's group): the codes are not computed
-            constexpr bool NOTB = decltype(keepv)::value == 2;
+            constexpr bool NOTB = KM == 2 || KM == 4;
+            // DTW over finite series: min / max folds, codes from equality tests
+            constexpr bool MXF = KM >= 3;
             const int j = s - l;
             const int tsym = cur.sym;
             [[maybe_unused]] float tv[SDD];
@@ -387,12 +390,23 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                     uint32_t bits = 0;
                     if constexpr (GAP == GAP_DTW) {
                         // D = opt(diag, up, left) + cost (solver.h:3236-3248)
-                        float m = Dd;
-                        int src = 1;
-                        if (better<DIR>(Dup, m)) { m = Dup; src = 2; }
-                        if (better<DIR>(Din[r], m)) { m = Din[r]; src = 3; }
-                        D = m + S;
-                        bits = uint32_t(src);
+                        if constexpr (MXF) {
+                            // finite series: every candidate is +0 or above
+                            // (or the worst value), never -0 or NaN, so the
+                            // fold's value is the strict fold's and its
+                            // choice the first candidate equal to it
+                            const float m = DIR == DIR_MAX ? fmaxf(fmaxf(Dd, Dup), Din[r])
+                                                           : fminf(fminf(Dd, Dup), Din[r]);
+                            D = m + S;
+                            bits = Dd == m ? 1u : (Dup == m ? 2u : 3u);
+                        } else {
+                            float m = Dd;
+                            int src = 1;
+                            if (better<DIR>(Dup, m)) { m = Dup; src = 2; }
+                            if (better<DIR>(Din[r], m)) { m = Din[r]; src = 3; }
+                            D = m + S;
+                            bits = uint32_t(src);
+                        }
                     } else if constexpr (GAP == GAP_LIN) {
                         // [0], diag, up, left (solver.h:2841-2852)
                         const float dg = Dd + S;
@@ -535,7 +549,8 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         // the step loop, built once per kind of group so that the usual loop
         // carries no per-cell test: 0 codes stored, 1 codes and cell values
         // (a.dvals), 2 neither (the groups of a checkpointed forward keep
-        // strip rows only; their folds keep their order, their codes go)
+        // strip rows only; their folds keep their order, their codes go);
+        // 3 and 4 as 0 and 2 for DTW over finite series, with min / max folds
         auto sweep = [&](auto kv) {
             int s = 0;
             if constexpr (STAGED) {
@@ -559,9 +574,21 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
             }
         };
         if constexpr (TR == TR_ONE) {
-            if (a.dvals != nullptr) sweep(integral_constant<int, 1>{});
-            else if (a.probs[g.first].tb_off == NO_TB_OFF) sweep(integral_constant<int, 2>{});  // (one problem a group)
-            else sweep(integral_constant<int, 0>{});
+            const bool notb = a.probs[g.first].tb_off == NO_TB_OFF;  // (one problem a group)
+            bool mx = false;
+            if constexpr (GAP == GAP_DTW && FORM == F_SERIES) mx = a.finite != nullptr && *a.finite != 0u;
+            if (a.dvals != nullptr) {
+                sweep(integral_constant<int, 1>{});
+            } else if (mx) {
+                if constexpr (GAP == GAP_DTW && FORM == F_SERIES) {
+                    if (notb) sweep(integral_constant<int, 4>{});
+                    else sweep(integral_constant<int, 3>{});
+                }
+            } else if (notb) {
+                sweep(integral_constant<int, 2>{});
+            } else {
+                sweep(integral_constant<int, 0>{});
+            }
         } else if constexpr (TR != TR_SCORE) {
             if (a.dvals != nullptr) sweep(integral_constant<int, 1>{});
             else sweep(integral_constant<int, 0>{});

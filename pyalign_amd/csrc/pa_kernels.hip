@@ -1531,6 +1531,26 @@ hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t value, hipStream_t s
 
 hipError_t launch_zero_u32(uint32_t *p, int64_t n, hipStream_t st) { return launch_fill_u32(p, n, 0u, st); }
 
+// flag (set to 1 beforehand) cleared when a value is not finite
+__global__ __launch_bounds__(256) void pa_all_finite_kernel(const float *x, int64_t nx, const float *y, int64_t ny,
+                                                            uint32_t *flag) {
+    bool bad = false;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nx + ny; k += stride)
+        bad = bad || !__builtin_isfinite(k < nx ? x[k] : y[k - nx]);
+    if (__any(bad) && (threadIdx.x & (WAVE - 1)) == 0)
+        __hip_atomic_store(flag, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+hipError_t launch_all_finite(const float *x, int64_t nx, const float *y, int64_t ny, uint32_t *flag,
+                             hipStream_t st) {
+    hipError_t e = launch_fill_u32(flag, 1, 1u, st);
+    if (e != hipSuccess || nx + ny == 0) return e;
+    const int64_t blocks = std::min<int64_t>((nx + ny + 255) / 256, int64_t(device_cus()) * 8);
+    hipLaunchKernelGGL(pa_all_finite_kernel, dim3(unsigned(blocks)), dim3(256), 0, st, x, nx, y, ny, flag);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Device memory of the allocation-free entry point: every GPU the library
 // runs on holds one copy of this array for the life of the process.  Host

@@ -366,6 +366,33 @@ def test_dtw_series_long_multi_strip():
         assert np.array_equal(np.asarray(r.path, dtype=np.int32), ref.path())
 
 
+@pytest.mark.parametrize("case", ["ties", "inf", "nan"])
+def test_dtw_series_min_folds_and_non_finite(case):
+    """Series of small integers (many equal candidates): over finite series
+    the kernels fold with min and take each code from equality tests, which
+    must give the oracle's choices; a bag holding an infinite or NaN value
+    keeps the strict compare-and-select folds, also the oracle's."""
+    rng = np.random.default_rng({"ties": 61, "inf": 62, "nan": 63}[case])
+    pf = pa.series(direction="minimize")
+    lens = [int(rng.integers(3, 160)) for _ in range(19)] + [700]
+    data = [(rng.integers(-2, 3, size=(a, 2)).astype(np.float32),
+             rng.integers(-2, 3, size=(int(rng.integers(3, 160)), 2)).astype(np.float32)) for a in lens]
+    if case == "inf":
+        data[3][0][2, 1] = np.inf
+    elif case == "nan":
+        data[5][1][1, 0] = np.nan
+    probs = [pf.new_problem(x, y) for x, y in data]
+    lims = (rounded(max(len(x) for x, _ in data)), rounded(max(len(y) for _, y in data)))
+    res = pa.ElasticSolver(codomain=pa.Solution).solve(probs)
+    for k, (p, r) in enumerate(zip(probs, res)):
+        ref = _dtw_oracle(p.matrix, O.MINIMIZE, O.GOAL_ONE, lims)
+        if np.isnan(ref.score):
+            assert np.isnan(r.score), k
+        else:
+            assert np.float32(r.score).tobytes() == ref.score.tobytes(), k
+        assert np.array_equal(np.asarray(r.path, dtype=np.int32), ref.path()), k
+
+
 def test_c2_full_length_range_against_oracle():
     """The batched local Gotoh workload's generator over its whole length
     range, [50, 2000]: 4,096 pairs solved in one bag on the fast kernel
