@@ -592,6 +592,9 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         } else if constexpr (TR != TR_SCORE) {
             if (a.dvals != nullptr) sweep(integral_constant<int, 1>{});
             else sweep(integral_constant<int, 0>{});
+        } else if constexpr (GAP == GAP_DTW && FORM == F_SERIES) {
+            if (a.finite != nullptr && *a.finite != 0u) sweep(integral_constant<int, 3>{});  // (no codes to compute)
+            else sweep(integral_constant<int, 0>{});
         } else {
             sweep(integral_constant<int, 0>{});
         }

@@ -384,12 +384,14 @@ def test_dtw_series_min_folds_and_non_finite(case):
     probs = [pf.new_problem(x, y) for x, y in data]
     lims = (rounded(max(len(x) for x, _ in data)), rounded(max(len(y) for _, y in data)))
     res = pa.ElasticSolver(codomain=pa.Solution).solve(probs)
-    for k, (p, r) in enumerate(zip(probs, res)):
+    scores = pa.ElasticSolver(codomain=pa.Score).solve(probs)
+    for k, (p, r, sc) in enumerate(zip(probs, res, scores)):
         ref = _dtw_oracle(p.matrix, O.MINIMIZE, O.GOAL_ONE, lims)
         if np.isnan(ref.score):
-            assert np.isnan(r.score), k
+            assert np.isnan(r.score) and np.isnan(sc), k
         else:
             assert np.float32(r.score).tobytes() == ref.score.tobytes(), k
+            assert np.float32(sc).tobytes() == ref.score.tobytes(), k
         assert np.array_equal(np.asarray(r.path, dtype=np.int32), ref.path()), k
 
 
