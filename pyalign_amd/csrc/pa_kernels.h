@@ -741,9 +741,22 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
     const uint8_t *__restrict__ cs = (const uint8_t *)a.codes_s;
     const uint8_t *__restrict__ ct = (const uint8_t *)a.codes_t;
     float best = NEG;
+    // steady steps: the members' shortest length and fewest strips, and safe
+    // symbol addresses (the first member's) for lanes without a member
+    int min_lt = has ? lt : 0x7fffffff, min_nst = has ? nst : 0x7fffffff;
+    for (int m = 1; m < WAVE; m <<= 1) {
+        min_lt = min(min_lt, __shfl_xor(min_lt, m, WAVE));
+        min_nst = min(min_nst, __shfl_xor(min_nst, m, WAVE));
+    }
+    min_lt = __builtin_amdgcn_readfirstlane(min_lt);  // the same in every lane
+    min_nst = __builtin_amdgcn_readfirstlane(min_nst);
+    // index of the symbol of column s - l + 1 is t_ix + s
+    const int64_t t_ix = (has ? off_t : a.probs[g.first].off_t) + 1 - l;
 
     for (int st = 0; st < g.max_nst; ++st) {
         const bool act = st < nst;
+        const bool rd_top = has && l == 0 && st > 0;         // lane 0 reads the strip above's row
+        const bool wr_bot = l == L - 1 && st + 1 < nst;      // the last lane writes it for the next
         const int row0 = st * rps + l * R;
         int rowsym[R];
         [[maybe_unused]] int rowpair[R / 2 > 0 ? R / 2 : 1];
@@ -781,19 +794,33 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
         int pendj = 0;
 
         // one step: column j = s - l of this lane's R rows, left column read
-        // from Din, this column written to Dout
-        auto step = [&](int s, float (&Din)[R], float (&Dout)[R]) {
+        // from Din, this column written to Dout.  A steady step (every lane of
+        // every member inside its problem's strip and columns, the next column
+        // too) drops the range tests; lanes without a member read the first
+        // member's symbols and discard what they compute.
+        auto step = [&](auto steady, int s, float (&Din)[R], float (&Dout)[R]) {
+            constexpr bool STEADY = decltype(steady)::value;
             const int j = s - l;
             const int tsym = tnext;
             const float rD = nxD, rP = nxP;
-            if (act && j + 1 >= 0 && j + 1 < lt) tnext = int(ct[off_t + j + 1]);
-            if (l == 0 && st > 0 && act && j + 1 < lt) {
-                const float2 v = a.rowbuf[rb_off + j + 1];
-                nxD = v.x;
-                nxP = v.y;
+            if constexpr (STEADY) {
+                tnext = int(ct[t_ix + s]);
+                if (rd_top) {
+                    const float2 v = a.rowbuf[rb_off + 1 + s];  // lane 0: column s + 1
+                    nxD = v.x;
+                    nxP = v.y;
+                }
+            } else {
+                if (act && j + 1 >= 0 && j + 1 < lt) tnext = int(ct[off_t + j + 1]);
+                if (l == 0 && st > 0 && act && j + 1 < lt) {
+                    const float2 v = a.rowbuf[rb_off + j + 1];
+                    nxD = v.x;
+                    nxP = v.y;
+                }
             }
             if (pend) {  // after the loads: the next wait on them need not wait for it
-                a.rowbuf[rb_off + pendj] = pendv;
+                if constexpr (STEADY) a.rowbuf[rb_off - 1 - l + s] = pendv;  // column s - 1 - l
+                else a.rowbuf[rb_off + pendj] = pendv;
                 pend = false;
             }
             float inD = shr1(botD), inP = shr1(botP);
@@ -803,7 +830,7 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
             }
             const float Ddiag = prevD;
             prevD = inD;
-            if (act && j >= 0 && j < lt) {
+            if (STEADY || (act && j >= 0 && j < lt)) {
                 float Dd = Ddiag, Pup = inP;
                 float Gup = inD + go_s;
                 [[maybe_unused]] float Sv[R];
@@ -840,19 +867,41 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
                 }
                 botD = Dout[R - 1];
                 botP = Pup;
-                if (l == L - 1 && st + 1 < nst) {
+                if constexpr (STEADY) {  // pendv is read only where pend is set
+                    pend = wr_bot;
+                    pendv = make_float2(botD, botP);
+                } else if (wr_bot) {
                     pend = true;
                     pendv = make_float2(botD, botP);
                     pendj = j;
                 }
             }
         };
+        using General = std::integral_constant<bool, false>;
+        using Steady = std::integral_constant<bool, true>;
+        // steps [sa, sb) are steady: s >= L - 1 puts every lane at a column
+        // >= 0, s + 1 < the members' shortest length keeps every lane's next
+        // column inside its member (only when every member still has this
+        // strip); both ends are even so the register banks keep their order
+        const int sa = L & ~1;
+        const int sb = st < min_nst ? ((min_lt - 1) & ~1) : 0;
         int s = 0;
-        for (; s + 1 < steps; s += 2) {
-            step(s, DlA, DlB);
-            step(s + 1, DlB, DlA);
+        if (sa < sb) {
+            for (; s < sa; s += 2) {
+                step(General{}, s, DlA, DlB);
+                step(General{}, s + 1, DlB, DlA);
+            }
+            for (int k = (sb - sa) >> 1; k > 0; --k, s += 2) {  // a uniform trip count
+                step(Steady{}, s, DlA, DlB);
+                step(Steady{}, s + 1, DlB, DlA);
+            }
+            pendj = s - 1 - l;  // the column of a store still pending
         }
-        if (s < steps) step(s, DlA, DlB);
+        for (; s + 1 < steps; s += 2) {
+            step(General{}, s, DlA, DlB);
+            step(General{}, s + 1, DlB, DlA);
+        }
+        if (s < steps) step(General{}, s, DlA, DlB);
         if (pend) a.rowbuf[rb_off + pendj] = pendv;
         __threadfence_block();
     }
@@ -881,7 +930,7 @@ __global__ __launch_bounds__(FAST_BLOCK) void pa_fast_kernel(KArgs a) {
     while (true) {
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(a.queue, 1u);
-        t = uint32_t(__shfl(int(t), 0, WAVE));
+        t = uint32_t(__builtin_amdgcn_readlane(int(t), 0));  // uniform: the group's fields load as scalars
         if (t >= uint32_t(a.n_groups)) break;
         const Group &g = a.groups[t];
         const int R = g.R;
