@@ -783,12 +783,10 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
         // of cells to complete before anything waits on it
         int tnext = 0;
         if (act && l == 0 && lt > 0) tnext = int(ct[off_t]);
-        float nxD = 0.0f, nxP = NEG;  // D(-1, j) = +0, P(-1, j) = worst on the first strip
-        if (l == 0 && st > 0 && act && lt > 0) {
-            const float2 v = a.rowbuf[rb_off];
-            nxD = v.x;
-            nxP = v.y;
-        }
+        // lane 0's row above, loaded a step ahead into alternating registers
+        // (a step reads one and loads the other: no copies between steps)
+        float2 nxA = make_float2(0.0f, NEG), nxB = nxA;  // D(-1, j) = +0, P(-1, j) = worst on the first strip
+        if (l == 0 && st > 0 && act && lt > 0) nxA = a.rowbuf[rb_off];
         bool pend = false;  // a bottom row value waiting to be stored
         float2 pendv = make_float2(0.0f, 0.0f);
         int pendj = 0;
@@ -798,25 +796,17 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
         // every member inside its problem's strip and columns, the next column
         // too) drops the range tests; lanes without a member read the first
         // member's symbols and discard what they compute.
-        auto step = [&](auto steady, int s, float (&Din)[R], float (&Dout)[R]) {
+        auto step = [&](auto steady, int s, float (&Din)[R], float (&Dout)[R], const float2 &nxin, float2 &nxout) {
             constexpr bool STEADY = decltype(steady)::value;
             const int j = s - l;
             const int tsym = tnext;
-            const float rD = nxD, rP = nxP;
+            const float rD = nxin.x, rP = nxin.y;
             if constexpr (STEADY) {
                 tnext = int(ct[t_ix + s]);
-                if (rd_top) {
-                    const float2 v = a.rowbuf[rb_off + 1 + s];  // lane 0: column s + 1
-                    nxD = v.x;
-                    nxP = v.y;
-                }
+                if (rd_top) nxout = a.rowbuf[rb_off + 1 + s];  // lane 0: column s + 1
             } else {
                 if (act && j + 1 >= 0 && j + 1 < lt) tnext = int(ct[off_t + j + 1]);
-                if (l == 0 && st > 0 && act && j + 1 < lt) {
-                    const float2 v = a.rowbuf[rb_off + j + 1];
-                    nxD = v.x;
-                    nxP = v.y;
-                }
+                if (l == 0 && st > 0 && act && j + 1 < lt) nxout = a.rowbuf[rb_off + j + 1];
             }
             if (pend) {  // after the loads: the next wait on them need not wait for it
                 if constexpr (STEADY) a.rowbuf[rb_off - 1 - l + s] = pendv;  // column s - 1 - l
@@ -888,20 +878,20 @@ __device__ __forceinline__ void solve_group_fast(const KArgs &a, const Group &g,
         int s = 0;
         if (sa < sb) {
             for (; s < sa; s += 2) {
-                step(General{}, s, DlA, DlB);
-                step(General{}, s + 1, DlB, DlA);
+                step(General{}, s, DlA, DlB, nxA, nxB);
+                step(General{}, s + 1, DlB, DlA, nxB, nxA);
             }
             for (int k = (sb - sa) >> 1; k > 0; --k, s += 2) {  // a uniform trip count
-                step(Steady{}, s, DlA, DlB);
-                step(Steady{}, s + 1, DlB, DlA);
+                step(Steady{}, s, DlA, DlB, nxA, nxB);
+                step(Steady{}, s + 1, DlB, DlA, nxB, nxA);
             }
             pendj = s - 1 - l;  // the column of a store still pending
         }
         for (; s + 1 < steps; s += 2) {
-            step(General{}, s, DlA, DlB);
-            step(General{}, s + 1, DlB, DlA);
+            step(General{}, s, DlA, DlB, nxA, nxB);
+            step(General{}, s + 1, DlB, DlA, nxB, nxA);
         }
-        if (s < steps) step(General{}, s, DlA, DlB);
+        if (s < steps) step(General{}, s, DlA, DlB, nxA, nxB);
         if (pend) a.rowbuf[rb_off + pendj] = pendv;
         __threadfence_block();
     }
