@@ -964,7 +964,7 @@ __global__ __launch_bounds__(256) void pa_solve_kernel(KArgs a) {
     while (true) {
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(a.queue, 1u);
-        t = uint32_t(__shfl(int(t), 0, WAVE));
+        t = uint32_t(__builtin_amdgcn_readlane(int(t), 0));  // uniform: the group's fields load as scalars
         if (t >= uint32_t(a.n_groups)) break;
         const Group &g = a.groups[t];
         if constexpr (!FAST && TR != TR_ALL) {
