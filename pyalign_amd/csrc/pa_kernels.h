@@ -108,7 +108,8 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
     if (has) {
         pr = a.probs[g.first + seg];
     } else {
-        pr.off_s = pr.off_t = pr.tb_off = pr.rb_off = pr.dv_off = 0;
+        pr.off_s = pr.off_t = pr.rb_off = pr.dv_off = 0;
+        pr.tb_off = NO_TB_OFF;  // nothing stored, also by steps that compute unguarded
         pr.len_s = pr.len_t = 0;
         pr.si = pr.sj = pr.out = 0;
         pr.band = -1;
@@ -127,6 +128,14 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
     float rbest = W, cbest = W;
     int rj = -1, ci = -1;
     float gscore = W;
+    // steady steps: the members' shortest length and fewest strips
+    int min_lt = has ? lt : 0x7fffffff, min_nst = has ? nst : 0x7fffffff;
+    for (int m = 1; m < WAVE; m <<= 1) {
+        min_lt = min(min_lt, __shfl_xor(min_lt, m, WAVE));
+        min_nst = min(min_nst, __shfl_xor(min_nst, m, WAVE));
+    }
+    min_lt = __builtin_amdgcn_readfirstlane(min_lt);  // the same in every lane
+    min_nst = __builtin_amdgcn_readfirstlane(min_nst);
     const int last = ls - 1;
     const int st_last = last >= 0 ? last / rps : -1;
     const int l_last = last >= 0 ? (last % rps) / R : -1;
@@ -316,7 +325,11 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         // D(i, j-1) is read from Din and this column written to Dout (two
         // banks used alternately: row r+1 still needs row r's old value as
         // its diagonal, so one bank would cost a register copy per cell)
-        auto step = [&](auto phase, auto keepv, int s, float (&Din)[R], float (&Dout)[R], Col &cur) {
+        // A steady step (every lane of every member inside its problem's strip
+        // and columns) computes its cells unguarded; lanes without a member
+        // compute from safe inputs and store nothing (no traceback offset).
+        auto step = [&](auto steady, auto phase, auto keepv, int s, float (&Din)[R], float (&Dout)[R], Col &cur) {
+            constexpr bool STEADY = decltype(steady)::value;
             constexpr int C = decltype(phase)::value;  // s % 8 (staged dense rows only)
             constexpr int KM = decltype(keepv)::value;
             constexpr bool KEEPV = KM == 1;  // cell values stored (a.dvals set)
@@ -359,7 +372,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
             flush();
             const float Ddiag = prevD;
             prevD = inD;
-            if (act && j >= 0 && j < lt) {
+            if (STEADY || (act && j >= 0 && j < lt)) {
                 float Dd = Ddiag, Dup = inD, Pup = inP;
                 uint32_t tbw[WPC];
 #pragma unroll
@@ -551,26 +564,46 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
         // (a.dvals), 2 neither (the groups of a checkpointed forward keep
         // strip rows only; their folds keep their order, their codes go);
         // 3 and 4 as 0 and 2 for DTW over finite series, with min / max folds
+        using Gen = integral_constant<bool, false>;
         auto sweep = [&](auto kv) {
             int s = 0;
             if constexpr (STAGED) {
                 // eight steps a round: the phase s % 8 is a compile-time constant
                 for (; s < steps; s += 8) {
-                    step(integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
-                    if (s + 1 < steps) step(integral_constant<int, 1>{}, kv, s + 1, DlB, DlA, cB);
-                    if (s + 2 < steps) step(integral_constant<int, 2>{}, kv, s + 2, DlA, DlB, cA);
-                    if (s + 3 < steps) step(integral_constant<int, 3>{}, kv, s + 3, DlB, DlA, cB);
-                    if (s + 4 < steps) step(integral_constant<int, 4>{}, kv, s + 4, DlA, DlB, cA);
-                    if (s + 5 < steps) step(integral_constant<int, 5>{}, kv, s + 5, DlB, DlA, cB);
-                    if (s + 6 < steps) step(integral_constant<int, 6>{}, kv, s + 6, DlA, DlB, cA);
-                    if (s + 7 < steps) step(integral_constant<int, 7>{}, kv, s + 7, DlB, DlA, cB);
+                    step(Gen{}, integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
+                    if (s + 1 < steps) step(Gen{}, integral_constant<int, 1>{}, kv, s + 1, DlB, DlA, cB);
+                    if (s + 2 < steps) step(Gen{}, integral_constant<int, 2>{}, kv, s + 2, DlA, DlB, cA);
+                    if (s + 3 < steps) step(Gen{}, integral_constant<int, 3>{}, kv, s + 3, DlB, DlA, cB);
+                    if (s + 4 < steps) step(Gen{}, integral_constant<int, 4>{}, kv, s + 4, DlA, DlB, cA);
+                    if (s + 5 < steps) step(Gen{}, integral_constant<int, 5>{}, kv, s + 5, DlB, DlA, cB);
+                    if (s + 6 < steps) step(Gen{}, integral_constant<int, 6>{}, kv, s + 6, DlA, DlB, cA);
+                    if (s + 7 < steps) step(Gen{}, integral_constant<int, 7>{}, kv, s + 7, DlB, DlA, cB);
                 }
             } else {
-                for (; s + 1 < steps; s += 2) {
-                    step(integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
-                    step(integral_constant<int, 0>{}, kv, s + 1, DlB, DlA, cB);
+                if constexpr (!BAND) {
+                    // steps [sa, sb) are steady: s >= L - 1 puts every lane at a
+                    // column >= 0, s < the members' shortest length keeps every
+                    // lane inside its member (only when every member still has
+                    // this strip); both ends even, so the banks keep their order
+                    const int sa = L & ~1;
+                    const int sb = st < min_nst ? (min_lt & ~1) : 0;
+                    if (sa < sb) {
+                        for (; s < sa; s += 2) {
+                            step(Gen{}, integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
+                            step(Gen{}, integral_constant<int, 0>{}, kv, s + 1, DlB, DlA, cB);
+                        }
+                        for (int k = (sb - sa) >> 1; k > 0; --k, s += 2) {  // a uniform trip count
+                            step(integral_constant<bool, true>{}, integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
+                            step(integral_constant<bool, true>{}, integral_constant<int, 0>{}, kv, s + 1, DlB, DlA,
+                                 cB);
+                        }
+                    }
                 }
-                if (s < steps) step(integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
+                for (; s + 1 < steps; s += 2) {
+                    step(Gen{}, integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
+                    step(Gen{}, integral_constant<int, 0>{}, kv, s + 1, DlB, DlA, cB);
+                }
+                if (s < steps) step(Gen{}, integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
             }
         };
         if constexpr (TR == TR_ONE) {
