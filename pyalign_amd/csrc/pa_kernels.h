@@ -228,9 +228,16 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
             float D, P;
         };
         Col cA, cB;
-        auto fetch = [&](int s, Col &c) {
+        // steady fetches: the column needs no clamp (the steady bounds keep it
+        // inside every member), lanes without a member read index 0, and the
+        // row-above strides (0, 1 or 2 floats) are a mask and a shift
+        const int fmask = act ? -1 : 0;
+        const int upMD = upDs != 0 ? -1 : 0, upSD = upDs >> 1;
+        const int upMP = upPs != 0 ? -1 : 0, upSP = upPs >> 1;
+        auto fetch = [&](auto steady, int s, Col &c) {
+            constexpr bool STEADY = decltype(steady)::value;
             const int j = s - l;
-            const int jc = act ? (j < 0 ? 0 : (j < lt ? j : lt - 1)) : 0;
+            const int jc = STEADY ? (j & fmask) : (act ? (j < 0 ? 0 : (j < lt ? j : lt - 1)) : 0);
             if constexpr (FORM == F_TABLE) {
                 c.sym = int(tb8[jc]);
             } else if constexpr (FORM == F_TABLE_G) {
@@ -248,8 +255,13 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 if (a.band_sync && l == 0 && act && st > 0 && j < lt)
                     band_wait(a, pr.band + st - 1, uint32_t(j + 1), seen);
             }
-            c.D = upD[jc * upDs];
-            c.P = upP[jc * upPs];
+            if constexpr (STEADY) {
+                c.D = upD[(jc & upMD) << upSD];
+                c.P = upP[(jc & upMP) << upSP];
+            } else {
+                c.D = upD[jc * upDs];
+                c.P = upP[jc * upPs];
+            }
         };
 
         // Stores of a step (the traceback words and the last lane's bottom
@@ -283,8 +295,8 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 pend_pub = false;
             }
         };
-        fetch(0, cA);
-        fetch(1, cB);
+        fetch(std::integral_constant<bool, false>{}, 0, cA);
+        fetch(std::integral_constant<bool, false>{}, 1, cB);
 
         // F_DROWS: window w holds this lane's R rows at columns [8w - l,
         // 8w - l + 8), read from global memory with two 16-byte loads per row
@@ -368,7 +380,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                 inD = cur.D;
                 inP = cur.P;
             }
-            fetch(s + 2, cur);
+            fetch(steady, s + 2, cur);
             flush();
             const float Ddiag = prevD;
             prevD = inD;
@@ -380,14 +392,21 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
 
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    // similarity S(i, j) (factory.h:573-576, 624-630, 673-679)
+                    // similarity S(i, j) (factory.h:573-576, 624-630, 673-679).
+                    // A padded row (i >= len_s, the last strip's end) feeds only
+                    // padded rows below it, stores nothing and enters no seed
+                    // search (the searches test rv), so its value is free except
+                    // for the score-only local maximum, which folds every row:
+                    // there it reads the worst similarity.
+                    constexpr bool PADW = LOC == LOC_LOCAL && TR == TR_SCORE;
                     float S;
                     if constexpr (TABLE) {
                         S = tab[rowsym[r] + tsym];
                     } else if constexpr (FORM == F_BINARY) {
-                        S = rv[r] ? (rowsym[r] == tsym ? a.eq : a.ne) : W;
+                        S = rowsym[r] == tsym ? a.eq : a.ne;
+                        if constexpr (PADW) S = rv[r] ? S : W;
                     } else if constexpr (DENSE) {
-                        S = rv[r] ? Scol[r] : W;
+                        S = PADW ? (rv[r] ? Scol[r] : W) : Scol[r];
                     } else {
                         // squared Euclidean cost, dimensions summed in order
                         float c = 0.0f;
@@ -396,7 +415,7 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
                             const float df = sa[r][d] - tv[d];
                             c = (d == 0) ? df * df : c + df * df;
                         }
-                        S = rv[r] ? c : W;
+                        S = PADW ? (rv[r] ? c : W) : c;
                     }
 
                     float D, P = 0.0f, Q = 0.0f;
@@ -582,11 +601,12 @@ __device__ __forceinline__ void solve_group(const KArgs &a, const Group &g, cons
             } else {
                 if constexpr (!BAND) {
                     // steps [sa, sb) are steady: s >= L - 1 puts every lane at a
-                    // column >= 0, s < the members' shortest length keeps every
-                    // lane inside its member (only when every member still has
-                    // this strip); both ends even, so the banks keep their order
+                    // column >= 0, s + 2 < the members' shortest length keeps
+                    // every lane's column and the one it fetches inside its
+                    // member (only when every member still has this strip); both
+                    // ends even, so the banks keep their order
                     const int sa = L & ~1;
-                    const int sb = st < min_nst ? (min_lt & ~1) : 0;
+                    const int sb = st < min_nst ? ((min_lt - 2) & ~1) : 0;
                     if (sa < sb) {
                         for (; s < sa; s += 2) {
                             step(Gen{}, integral_constant<int, 0>{}, kv, s, DlA, DlB, cA);
